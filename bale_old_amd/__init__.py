@@ -1,0 +1,201 @@
+"""bale_old_amd -- MI355X (gfx950) histogram scatter-add and index gather.
+
+The product is ``libbale_hip.so`` (C-ABI declared in ``include/bale_hip.h``).
+This module only loads it with ctypes and passes torch tensors' device
+pointers and streams through; it never computes a result itself.  If the
+library is missing, :func:`lib` raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libbale_hip.so"
+
+# error codes (include/bale_hip.h)
+OK, ERR_ARG, ERR_RANGE, ERR_HIP, ERR_ALLOC, ERR_PE, ERR_NODEV = 0, -1, -2, -3, -4, -5, -6
+# strategies / dtypes / formats / merge modes
+AUTO, DIRECT, LDS, BUCKET = 0, 1, 2, 3
+I64, I32 = 0, 1
+PLAIN, PACKED = 0, 1
+MERGE_NONE, MERGE_RUNS, MERGE_MATCH = 0, 1, 2
+
+STRATEGY_NAMES = {AUTO: "auto", DIRECT: "direct", LDS: "lds", BUCKET: "bucket"}
+
+
+class BaleConfig(C.Structure):
+    _fields_ = [
+        ("strategy", C.c_int32),
+        ("block", C.c_int32),
+        ("unroll", C.c_int32),
+        ("tile_log2", C.c_int32),
+        ("merge", C.c_int32),
+        ("groups_per_cu", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+    def __repr__(self) -> str:
+        return f"BaleConfig({self.as_dict()})"
+
+
+def config(strategy=AUTO, block=0, unroll=0, tile_log2=0, merge=-1, groups_per_cu=0) -> BaleConfig:
+    return BaleConfig(strategy, block, unroll, tile_log2, merge, groups_per_cu)
+
+
+class BaleError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bale error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+P64 = C.POINTER(C.c_int64)
+VP = C.c_void_p
+I64T = C.c_int64
+DROPIN_HISTO = ["histo_agi"]
+DROPIN_HISTO_BUF = ["histo_exstack", "histo_exstack2", "histo_exstack2_goto", "histo_exstack2_cyclic",
+                    "histo_exstack_function", "histo_exstack2_function"]
+DROPIN_IG_BUF = ["ig_exstack", "ig_exstack2", "ig_exstack2_cyclic", "ig_exstack2_goto",
+                 "ig_exstack_function", "ig_exstack_pkg"]
+
+
+def _proto(L) -> None:
+    def f(name, res, *args):
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = list(args)
+
+    f("bale_hip_init", C.c_int, C.c_int)
+    f("bale_hip_last_error", C.c_char_p)
+    f("bale_hip_last_timing", None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double))
+    f("bale_hip_device_info", C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
+    f("bale_histo", C.c_int, VP, I64T, VP, I64T, C.c_int, C.c_int, C.POINTER(BaleConfig), VP)
+    f("bale_gather", C.c_int, VP, VP, I64T, VP, I64T, C.c_int, C.POINTER(BaleConfig), VP)
+    f("bale_sync_errors", C.c_int, VP, P64)
+    f("bale_histo_plan", C.c_int, I64T, I64T, C.c_int, C.POINTER(BaleConfig))
+    f("bale_gather_plan", C.c_int, I64T, I64T, C.POINTER(BaleConfig))
+    f("bale_fill_splitmix", C.c_int, VP, I64T, VP)
+    f("bale_fill_neg", C.c_int, VP, I64T, VP)
+    f("bale_gen_index", C.c_int, VP, I64T, I64T, C.c_int, C.c_double, I64T, C.c_uint64, VP)
+    f("bale_pack", C.c_int, VP, VP, I64T, VP)
+    f("bale_probe_copy", C.c_int, VP, VP, I64T, VP)
+    f("bale_probe_random_read", C.c_int, VP, I64T, I64T, VP, VP)
+    f("histo_agi", C.c_double, VP, I64T, VP)
+    f("histo_conveyor", C.c_double, VP, I64T, VP)
+    for name in DROPIN_HISTO_BUF:
+        f(name, C.c_double, VP, I64T, VP, I64T)
+    f("ig_agi", C.c_double, VP, VP, I64T, VP)
+    f("ig_conveyor", C.c_double, VP, VP, I64T, VP)
+    for name in DROPIN_IG_BUF:
+        f(name, C.c_double, VP, VP, I64T, VP, I64T)
+
+
+def lib():
+    """The loaded libbale_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m bale_old_amd.build` "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        _proto(L)
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().bale_hip_last_error().decode()
+
+
+def check(code: int) -> int:
+    if code != OK:
+        raise BaleError(code, last_error())
+    return code
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    return t.data_ptr() if hasattr(t, "data_ptr") else int(t)
+
+
+def _stream(stream) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return stream if isinstance(stream, int) else stream.cuda_stream
+
+
+def device_info() -> tuple[int, int]:
+    cus, lds = C.c_int(), C.c_int()
+    check(lib().bale_hip_device_info(C.byref(cus), C.byref(lds)))
+    return cus.value, lds.value
+
+
+def histo(index, counts, *, nbins=None, packed=False, cfg: BaleConfig | None = None, stream=None,
+          sync=True) -> int:
+    """counts[index[i]] += 1 on the GPU (torch tensors on cuda)."""
+    import torch
+    dtype = {torch.int64: I64, torch.int32: I32}[counts.dtype]
+    nb = counts.numel() if nbins is None else nbins
+    rc = lib().bale_histo(_ptr(index), index.numel(), _ptr(counts), nb, dtype,
+                          PACKED if packed else PLAIN, C.byref(cfg) if cfg else None, _stream(stream))
+    check(rc)
+    return sync_errors(stream) if sync else 0
+
+
+def gather(tgt, index, table, *, packed=False, cfg: BaleConfig | None = None, stream=None,
+           sync=True) -> int:
+    """tgt[i] = table[index[i]] on the GPU (torch tensors on cuda)."""
+    rc = lib().bale_gather(_ptr(tgt), _ptr(index), index.numel(), _ptr(table), table.numel(),
+                           PACKED if packed else PLAIN, C.byref(cfg) if cfg else None, _stream(stream))
+    check(rc)
+    return sync_errors(stream) if sync else 0
+
+
+def sync_errors(stream=None) -> int:
+    bad = C.c_int64(0)
+    check(lib().bale_sync_errors(_stream(stream), C.byref(bad)))
+    return bad.value
+
+
+def histo_plan(n: int, nbins: int, dtype: int = I64) -> BaleConfig:
+    c = BaleConfig()
+    check(lib().bale_histo_plan(n, nbins, dtype, C.byref(c)))
+    return c
+
+
+def gather_plan(n: int, table_len: int) -> BaleConfig:
+    c = BaleConfig()
+    check(lib().bale_gather_plan(n, table_len, C.byref(c)))
+    return c
+
+
+def fill_splitmix(table, stream=None) -> None:
+    check(lib().bale_fill_splitmix(_ptr(table), table.numel(), _stream(stream)))
+
+
+def fill_neg(table, stream=None) -> None:
+    check(lib().bale_fill_neg(_ptr(table), table.numel(), _stream(stream)))
+
+
+def gen_index(out, nbins: int, *, dist: int = 0, s: float = 1.0, run_mean: int = 0, seed: int = 1,
+              stream=None) -> None:
+    """Seeded index stream on the GPU (dist 0 uniform, 1 Zipf(s); run_mean>1 clusters)."""
+    check(lib().bale_gen_index(_ptr(out), out.numel(), nbins, dist, float(s), run_mean,
+                               seed & 0xFFFFFFFFFFFFFFFF, _stream(stream)))
+
+
+def pack(pck, index, stream=None) -> None:
+    check(lib().bale_pack(_ptr(pck), _ptr(index), index.numel(), _stream(stream)))
+
+
+def last_timing() -> tuple[float, float, float]:
+    g, h, d = C.c_double(), C.c_double(), C.c_double()
+    lib().bale_hip_last_timing(C.byref(g), C.byref(h), C.byref(d))
+    return g.value, h.value, d.value

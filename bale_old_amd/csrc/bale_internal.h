@@ -1,0 +1,121 @@
+// bale_internal.h -- shared device helpers and launcher declarations for
+// libbale_hip.so (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bale_hip.h"
+
+namespace bale {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ULL;
+
+// 16-byte vector types the non-temporal builtins accept
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef long long i64x2 __attribute__((ext_vector_type(2)));
+
+// ---- splitmix64 (Steele, Lea, Flood 2014) ---------------------------------
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+// i-th output (0-based) of the stream seeded with `seed`
+__host__ __device__ inline uint64_t stream_at(uint64_t seed, uint64_t i) {
+  return mix64(seed + (i + 1) * kGolden);
+}
+__device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+// [0,n) from a 64-bit draw: top bits for powers of two, else multiply-high
+__device__ inline uint64_t reduce_range(uint64_t x, uint64_t n) {
+  if (n <= 1) return 0;
+  if ((n & (n - 1)) == 0) return x >> (64 - __builtin_ctzll(n));
+  return mulhi64(x, n);
+}
+
+// ---- per-device error record ---------------------------------------------
+// err[0]: indices outside the table (skipped); err[1]: packed PE != 0.
+struct ErrWords {
+  unsigned long long range;
+  unsigned long long pe;
+};
+
+// Decode one index. Returns false (and records why) when it must be skipped.
+template <bool PACKED>
+__device__ inline bool decode_index(int64_t raw, uint64_t limit, uint64_t& j,
+                                    unsigned& nbad_range, unsigned& nbad_pe) {
+  int64_t v = raw;
+  if (PACKED) {
+    if ((raw & 0xffff) != 0) { nbad_pe++; return false; }
+    v = raw >> 16;
+  }
+  j = (uint64_t)v;
+  if (j >= limit) { nbad_range++; return false; }
+  return true;
+}
+
+__device__ inline void flush_errors(ErrWords* err, unsigned nbad_range, unsigned nbad_pe) {
+  if (nbad_range) atomicAdd(&err->range, (unsigned long long)nbad_range);
+  if (nbad_pe) atomicAdd(&err->pe, (unsigned long long)nbad_pe);
+}
+
+// device-scope relaxed add without return (global_atomic_add_x2 / _u32)
+__device__ inline void atomic_add_dev(int64_t* p, int64_t v) {
+  __hip_atomic_fetch_add((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void atomic_add_dev(int32_t* p, int32_t v) {
+  __hip_atomic_fetch_add((unsigned*)p, (unsigned)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Raise a kernel's dynamic-LDS limit before a launch that needs more than the
+// default 64 KiB.
+template <typename K>
+inline void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
+}
+
+// ---- per-device state (capi.hip) -----------------------------------------
+struct Device {
+  int id = -1;
+  int cus = 0;
+  int lds_per_block = 0;
+  ErrWords* err = nullptr;    // device error record
+  void* ws = nullptr;         // grow-only workspace
+  size_t ws_bytes = 0;
+};
+Device* device_state();  // current device's state (created on first use)
+// Workspace of at least `bytes`, 256-byte aligned; nullptr on failure.
+void* workspace(Device* d, size_t bytes, hipStream_t s);
+
+// ---- launchers (return hipError_t-like int, 0 = ok) -----------------------
+int launch_fill_splitmix(int64_t* t, int64_t n, hipStream_t s);
+int launch_fill_neg(int64_t* t, int64_t n, hipStream_t s);
+int launch_gen_index(int64_t* out, int64_t n, int64_t nbins, int dist, double zs,
+                     int64_t run_mean, uint64_t seed, hipStream_t s);
+int launch_pack(int64_t* pck, const int64_t* idx, int64_t n, hipStream_t s);
+int launch_zero_bytes(void* p, size_t bytes, hipStream_t s);
+int launch_minmax(const int64_t* idx, int64_t n, int packed, int64_t* out2, hipStream_t s);
+
+int launch_gather_direct(int64_t* tgt, const int64_t* idx, int64_t n, const int64_t* table,
+                         int64_t table_len, bool packed, const bale_config& c, ErrWords* err,
+                         hipStream_t s);
+int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int64_t* table,
+                         int64_t table_len, bool packed, const bale_config& c, Device* d,
+                         hipStream_t s);
+
+int launch_histo_direct(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
+                        bool packed, const bale_config& c, ErrWords* err, hipStream_t s);
+int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
+                     bool packed, const bale_config& c, Device* d, hipStream_t s);
+int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
+                        bool packed, const bale_config& c, Device* d, hipStream_t s);
+
+int launch_probe_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
+int launch_probe_random(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
+                        hipStream_t s);
+
+}  // namespace bale

@@ -1,0 +1,520 @@
+// capi.hip -- the C-ABI of libbale_hip.so: per-device state, the native
+// bale_* API and the reference drop-ins (histo_*, ig_*).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "bale_internal.h"
+
+namespace bale {
+
+static thread_local std::string t_err;
+static thread_local double t_gpu_s = 0.0, t_h2d_s = 0.0, t_d2h_s = 0.0;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  return code;
+}
+static int fail_hip(int rc, const char* what) {
+  return fail(BALE_ERR_HIP, "%s: %s", what, hipGetErrorString((hipError_t)rc));
+}
+
+constexpr int kMaxDevices = 64;
+static Device g_dev[kMaxDevices];
+static std::mutex g_mu;
+
+Device* device_state() {
+  int id = 0;
+  if (hipGetDevice(&id) != hipSuccess || id < 0 || id >= kMaxDevices) return nullptr;
+  Device* d = &g_dev[id];
+  if (d->id == id) return d;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (d->id == id) return d;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, id) != hipSuccess) return nullptr;
+  d->cus = p.multiProcessorCount;
+  int lds = (int)p.sharedMemPerBlock;
+  int optin = 0;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, id) == hipSuccess &&
+      optin > lds)
+    lds = optin;
+  (void)hipGetLastError();
+  d->lds_per_block = lds;
+  if (hipMalloc((void**)&d->err, sizeof(ErrWords)) != hipSuccess) return nullptr;
+  if (hipMemset(d->err, 0, sizeof(ErrWords)) != hipSuccess) return nullptr;
+  if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+  d->id = id;
+  return d;
+}
+
+void* workspace(Device* d, size_t bytes, hipStream_t s) {
+  if (d->ws_bytes >= bytes) return d->ws;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (d->ws_bytes >= bytes) return d->ws;
+  if (d->ws) {
+    if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
+    (void)hipFree(d->ws);
+    d->ws = nullptr;
+    d->ws_bytes = 0;
+  }
+  size_t want = bytes + (bytes >> 3);  // grow with headroom
+  if (hipMalloc(&d->ws, want) != hipSuccess) {
+    (void)hipGetLastError();
+    d->ws = nullptr;
+    return nullptr;
+  }
+  d->ws_bytes = want;
+  return d->ws;
+}
+
+static int max_tile_bits_for(const Device* d) {
+  int tb = 15;
+  while (tb > 8 && ((size_t)4 << tb) + 1024 > (size_t)d->lds_per_block) tb--;
+  return tb;
+}
+
+// Default rule when no tuned configuration is given.
+static void plan_histo(const Device* d, int64_t n, int64_t nbins, int dtype, bale_config* c) {
+  memset(c, 0, sizeof *c);
+  c->merge = BALE_MERGE_NONE;
+  int tb = max_tile_bits_for(d);
+  int64_t bytes = nbins * (dtype == BALE_I64 ? 8 : 4);
+  if (nbins <= ((int64_t)1 << tb) && n >= 8 * nbins) {
+    c->strategy = BALE_STRAT_LDS;
+    c->block = 1024;
+    c->unroll = 8;
+  } else if (bytes > ((int64_t)64 << 20) && (nbins >> tb) <= 4096 && n >= nbins / 4) {
+    c->strategy = BALE_STRAT_BUCKET;
+    c->block = 1024;
+  } else {
+    c->strategy = BALE_STRAT_DIRECT;
+    c->block = 256;
+    c->unroll = 8;
+  }
+}
+static void plan_gather(const Device*, int64_t, int64_t, bale_config* c) {
+  memset(c, 0, sizeof *c);
+  c->strategy = BALE_STRAT_DIRECT;
+  c->block = 256;
+  c->unroll = 16;
+  c->merge = BALE_MERGE_NONE;
+}
+
+static bool is_device_ptr(const void* p) {
+  hipPointerAttribute_t a;
+  memset(&a, 0, sizeof a);
+  hipError_t rc = hipPointerGetAttributes(&a, p);
+  if (rc != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// Entries of int64 addressable from p to the end of its device allocation.
+static int64_t device_extent(const void* p, size_t elem) {
+  void* base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (void*)p) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  size_t off = (const char*)p - (const char*)base;
+  return off > size ? 0 : (int64_t)((size - off) / elem);
+}
+
+}  // namespace bale
+
+using namespace bale;
+
+extern "C" {
+
+int bale_hip_init(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return fail(BALE_ERR_NODEV, "no HIP device");
+  }
+  if (device < 0 || device >= n) return fail(BALE_ERR_ARG, "device %d out of range [0,%d)", device, n);
+  int rc = (int)hipSetDevice(device);
+  if (rc) return fail_hip(rc, "hipSetDevice");
+  if (!device_state()) return fail(BALE_ERR_HIP, "device state setup failed");
+  return BALE_OK;
+}
+
+const char* bale_hip_last_error(void) { return t_err.c_str(); }
+
+void bale_hip_last_timing(double* gpu_s, double* h2d_s, double* d2h_s) {
+  if (gpu_s) *gpu_s = t_gpu_s;
+  if (h2d_s) *h2d_s = t_h2d_s;
+  if (d2h_s) *d2h_s = t_d2h_s;
+}
+
+// cus, lds bytes per workgroup
+int bale_hip_device_info(int* cus, int* lds_per_block) {
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  if (cus) *cus = d->cus;
+  if (lds_per_block) *lds_per_block = d->lds_per_block;
+  return BALE_OK;
+}
+
+int bale_histo_plan(int64_t n, int64_t nbins, int dtype, bale_config* out) {
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  if (!out) return fail(BALE_ERR_ARG, "null config");
+  plan_histo(d, n, nbins, dtype, out);
+  return BALE_OK;
+}
+
+int bale_gather_plan(int64_t n, int64_t table_len, bale_config* out) {
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  if (!out) return fail(BALE_ERR_ARG, "null config");
+  plan_gather(d, n, table_len, out);
+  return BALE_OK;
+}
+
+int bale_histo(const int64_t* index, int64_t n, void* counts, int64_t nbins, int dtype,
+               int index_format, const bale_config* cfg, void* stream) {
+  if (n < 0) return fail(BALE_ERR_ARG, "negative update count %lld", (long long)n);
+  if (n == 0) return BALE_OK;
+  if (!index || !counts) return fail(BALE_ERR_ARG, "null index or counts");
+  if (nbins <= 0) return fail(BALE_ERR_ARG, "nbins must be positive");
+  if (dtype != BALE_I64 && dtype != BALE_I32) return fail(BALE_ERR_ARG, "bad dtype %d", dtype);
+  if (index_format != BALE_INDEX_PLAIN && index_format != BALE_INDEX_PACKED)
+    return fail(BALE_ERR_ARG, "bad index format %d", index_format);
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  hipStream_t s = (hipStream_t)stream;
+  bale_config c;
+  if (cfg && cfg->strategy != BALE_STRAT_AUTO) c = *cfg;
+  else plan_histo(d, n, nbins, dtype, &c);
+  bool packed = index_format == BALE_INDEX_PACKED;
+  // the bucketed pass keeps 32-bit positions: run it in slices
+  const int64_t slice = c.strategy == BALE_STRAT_BUCKET ? ((int64_t)1 << 30) : n;
+  for (int64_t off = 0; off < n; off += slice) {
+    int64_t m = std::min(slice, n - off);
+    int rc;
+    switch (c.strategy) {
+      case BALE_STRAT_DIRECT:
+        rc = launch_histo_direct(index + off, m, counts, nbins, dtype, packed, c, d->err, s);
+        break;
+      case BALE_STRAT_LDS:
+        rc = launch_histo_lds(index + off, m, counts, nbins, dtype, packed, c, d, s);
+        break;
+      case BALE_STRAT_BUCKET:
+        rc = launch_histo_bucket(index + off, m, counts, nbins, dtype, packed, c, d, s);
+        break;
+      default:
+        return fail(BALE_ERR_ARG, "bad histogram strategy %d", c.strategy);
+    }
+    if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "workspace allocation failed");
+    if (rc) return fail_hip(rc, "histogram launch");
+  }
+  return BALE_OK;
+}
+
+int bale_gather(int64_t* tgt, const int64_t* index, int64_t n, const int64_t* table,
+                int64_t table_len, int index_format, const bale_config* cfg, void* stream) {
+  if (n < 0) return fail(BALE_ERR_ARG, "negative request count %lld", (long long)n);
+  if (n == 0) return BALE_OK;
+  if (!tgt || !index || !table) return fail(BALE_ERR_ARG, "null tgt, index or table");
+  if (table_len <= 0) return fail(BALE_ERR_ARG, "table_len must be positive");
+  if (index_format != BALE_INDEX_PLAIN && index_format != BALE_INDEX_PACKED)
+    return fail(BALE_ERR_ARG, "bad index format %d", index_format);
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  hipStream_t s = (hipStream_t)stream;
+  bale_config c;
+  if (cfg && cfg->strategy != BALE_STRAT_AUTO) c = *cfg;
+  else plan_gather(d, n, table_len, &c);
+  bool packed = index_format == BALE_INDEX_PACKED;
+  const int64_t slice = c.strategy == BALE_STRAT_BUCKET ? ((int64_t)1 << 30) : n;
+  for (int64_t off = 0; off < n; off += slice) {
+    int64_t m = std::min(slice, n - off);
+    int rc;
+    switch (c.strategy) {
+      case BALE_STRAT_DIRECT:
+        rc = launch_gather_direct(tgt + off, index + off, m, table, table_len, packed, c, d->err, s);
+        break;
+      case BALE_STRAT_BUCKET:
+        rc = launch_gather_bucket(tgt + off, index + off, m, table, table_len, packed, c, d, s);
+        break;
+      default:
+        return fail(BALE_ERR_ARG, "bad gather strategy %d", c.strategy);
+    }
+    if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "workspace allocation failed");
+    if (rc) return fail_hip(rc, "gather launch");
+  }
+  return BALE_OK;
+}
+
+int bale_sync_errors(void* stream, int64_t* bad) {
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  int rc = (int)hipStreamSynchronize((hipStream_t)stream);
+  if (rc) return fail_hip(rc, "hipStreamSynchronize");
+  ErrWords h{0, 0}, z{0, 0};
+  rc = (int)hipMemcpy(&h, d->err, sizeof h, hipMemcpyDeviceToHost);
+  if (rc) return fail_hip(rc, "reading the error record");
+  if (h.range || h.pe) {
+    rc = (int)hipMemcpy(d->err, &z, sizeof z, hipMemcpyHostToDevice);
+    if (rc) return fail_hip(rc, "clearing the error record");
+  }
+  if (bad) *bad = (int64_t)(h.range + h.pe);
+  if (h.pe) return fail(BALE_ERR_PE, "%llu packed indices name a PE other than 0 (one GPU is PE 0)",
+                        (unsigned long long)h.pe);
+  if (h.range) return fail(BALE_ERR_RANGE, "%llu indices outside the table were skipped",
+                           (unsigned long long)h.range);
+  return BALE_OK;
+}
+
+int bale_fill_splitmix(int64_t* table, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && !table)) return fail(BALE_ERR_ARG, "bad table");
+  int rc = launch_fill_splitmix(table, n, (hipStream_t)stream);
+  return rc ? fail_hip(rc, "fill") : BALE_OK;
+}
+
+int bale_fill_neg(int64_t* table, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && !table)) return fail(BALE_ERR_ARG, "bad table");
+  int rc = launch_fill_neg(table, n, (hipStream_t)stream);
+  return rc ? fail_hip(rc, "fill") : BALE_OK;
+}
+
+int bale_gen_index(int64_t* out, int64_t n, int64_t nbins, int dist, double s, int64_t run_mean,
+                   uint64_t seed, void* stream) {
+  if (n < 0 || (n > 0 && !out)) return fail(BALE_ERR_ARG, "bad output");
+  if (nbins <= 0) return fail(BALE_ERR_ARG, "nbins must be positive");
+  if (dist != 0 && dist != 1) return fail(BALE_ERR_ARG, "dist must be 0 (uniform) or 1 (zipf)");
+  if (dist == 1 && !(s > 0.0 && s < 10.0)) return fail(BALE_ERR_ARG, "zipf exponent out of (0,10)");
+  int rc = launch_gen_index(out, n, nbins, dist, s, run_mean, seed, (hipStream_t)stream);
+  return rc ? fail_hip(rc, "gen_index") : BALE_OK;
+}
+
+int bale_pack(int64_t* pck, const int64_t* index, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && (!pck || !index))) return fail(BALE_ERR_ARG, "bad arrays");
+  int rc = launch_pack(pck, index, n, (hipStream_t)stream);
+  return rc ? fail_hip(rc, "pack") : BALE_OK;
+}
+
+int bale_probe_copy(void* dst, const void* src, int64_t bytes, void* stream) {
+  if (bytes < 0 || (bytes % 16) || (bytes && (!dst || !src)))
+    return fail(BALE_ERR_ARG, "copy size must be a multiple of 16");
+  int rc = launch_probe_copy(dst, src, bytes, (hipStream_t)stream);
+  return rc ? fail_hip(rc, "probe_copy") : BALE_OK;
+}
+
+int bale_probe_random_read(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
+                           void* stream) {
+  if (n <= 0 || nreads < 0 || !table || !out) return fail(BALE_ERR_ARG, "bad probe arguments");
+  int rc = launch_probe_random(table, n, nreads, out, (hipStream_t)stream);
+  return rc ? fail_hip(rc, "probe_random") : BALE_OK;
+}
+
+}  // extern "C"
+
+// ======================= reference drop-ins ===============================
+namespace bale {
+
+struct Staged {
+  void* p = nullptr;
+  bool owned = false;
+  ~Staged() {
+    if (owned && p) (void)hipFree(p);
+  }
+};
+
+static double elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+  return ms * 1e-3;
+}
+
+// Shared body of every drop-in.  kind 0: histogram (counts += ...),
+// kind 1: gather (tgt = table[...]).
+static double dropin(int kind, int64_t* out, int64_t* index, int64_t n, int64_t* table_or_counts,
+                     bool packed, int strategy, int merge) {
+  t_gpu_s = t_h2d_s = t_d2h_s = 0.0;
+  if (n < 0) { fail(BALE_ERR_ARG, "negative length"); return -1.0; }
+  if (n > 0 && (!index || !table_or_counts || (kind == 1 && !out))) {
+    fail(BALE_ERR_ARG, "null array");
+    return -1.0;
+  }
+  Device* d = device_state();
+  if (!d) { fail(BALE_ERR_NODEV, "no usable HIP device"); return -1.0; }
+  if (n == 0) return 0.0;
+  hipStream_t s = 0;
+  hipEvent_t e0, e1, e2, e3;
+  hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
+  auto cleanup = [&]() {
+    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
+  };
+  double result = -1.0;
+  Staged sidx, stab, sout;
+  int64_t tlen = -1;
+  int rc;
+  hipEventRecord(e0, s);
+  // index
+  if (is_device_ptr(index)) {
+    sidx.p = index;
+  } else {
+    if ((rc = (int)hipMalloc(&sidx.p, (size_t)n * 8))) { fail(BALE_ERR_ALLOC, "staging index"); cleanup(); return -1.0; }
+    sidx.owned = true;
+    if ((rc = (int)hipMemcpy(sidx.p, index, (size_t)n * 8, hipMemcpyHostToDevice))) { fail_hip(rc, "index H2D"); cleanup(); return -1.0; }
+  }
+  // table / counts: its needed length is one past the largest index passed
+  bool host_tab = !is_device_ptr(table_or_counts);
+  {
+    int64_t* mm = nullptr;
+    int64_t init[2] = {INT64_MAX, INT64_MIN}, got[2];
+    if (hipMalloc((void**)&mm, 16) != hipSuccess) { fail(BALE_ERR_ALLOC, "minmax"); cleanup(); return -1.0; }
+    (void)hipMemcpy(mm, init, 16, hipMemcpyHostToDevice);
+    launch_minmax((const int64_t*)sidx.p, n, packed ? 1 : 0, mm, s);
+    rc = (int)hipMemcpy(got, mm, 16, hipMemcpyDeviceToHost);
+    (void)hipFree(mm);
+    if (rc) { fail_hip(rc, "minmax"); cleanup(); return -1.0; }
+    if (got[0] < 0) { fail(BALE_ERR_RANGE, "negative index %lld", (long long)got[0]); cleanup(); return -1.0; }
+    tlen = got[1] + 1;
+  }
+  if (!host_tab) {
+    stab.p = table_or_counts;
+    int64_t ext = device_extent(table_or_counts, 8);
+    if (ext >= 0 && tlen > ext) {
+      fail(BALE_ERR_RANGE, "index %lld is past the end of the device allocation (%lld entries)",
+           (long long)(tlen - 1), (long long)ext);
+      cleanup();
+      return -1.0;
+    }
+  } else {
+    if (hipMalloc(&stab.p, (size_t)tlen * 8) != hipSuccess) { fail(BALE_ERR_ALLOC, "staging table"); cleanup(); return -1.0; }
+    stab.owned = true;
+    if ((rc = (int)hipMemcpy(stab.p, table_or_counts, (size_t)tlen * 8, hipMemcpyHostToDevice))) { fail_hip(rc, "table H2D"); cleanup(); return -1.0; }
+  }
+  // gather output
+  bool host_out = kind == 1 && !is_device_ptr(out);
+  if (kind == 1) {
+    if (host_out) {
+      if (hipMalloc(&sout.p, (size_t)n * 8) != hipSuccess) { fail(BALE_ERR_ALLOC, "staging tgt"); cleanup(); return -1.0; }
+      sout.owned = true;
+    } else {
+      sout.p = out;
+    }
+  }
+  hipEventRecord(e1, s);
+  bale_config c;
+  memset(&c, 0, sizeof c);
+  int r;
+  if (strategy == BALE_STRAT_AUTO) {
+    c.strategy = BALE_STRAT_AUTO;
+  } else {
+    if (kind == 0) plan_histo(d, n, tlen, BALE_I64, &c);
+    else plan_gather(d, n, tlen, &c);
+    c.strategy = strategy;
+    // beyond 4096 bucket tiles the bucketed histogram is not offered
+    if (kind == 0 && strategy == BALE_STRAT_BUCKET &&
+        ceil_div(tlen, (int64_t)1 << max_tile_bits_for(d)) > 4096)
+      c.strategy = BALE_STRAT_DIRECT;
+    if (strategy == BALE_STRAT_LDS || strategy == BALE_STRAT_BUCKET) c.block = 1024;
+    if (strategy == BALE_STRAT_LDS && c.unroll == 0) c.unroll = 8;
+    c.merge = merge;
+  }
+  if (kind == 0)
+    r = bale_histo((const int64_t*)sidx.p, n, stab.p, tlen, BALE_I64,
+                   packed ? BALE_INDEX_PACKED : BALE_INDEX_PLAIN, &c, (void*)s);
+  else
+    r = bale_gather((int64_t*)sout.p, (const int64_t*)sidx.p, n, (const int64_t*)stab.p, tlen,
+                    packed ? BALE_INDEX_PACKED : BALE_INDEX_PLAIN, &c, (void*)s);
+  hipEventRecord(e2, s);
+  if (r == BALE_OK) {
+    if (kind == 0 && host_tab)
+      rc = (int)hipMemcpy(table_or_counts, stab.p, (size_t)tlen * 8, hipMemcpyDeviceToHost);
+    else if (kind == 1 && host_out)
+      rc = (int)hipMemcpy(out, sout.p, (size_t)n * 8, hipMemcpyDeviceToHost);
+    else
+      rc = 0;
+    hipEventRecord(e3, s);
+    if (rc) r = fail_hip(rc, "result D2H");
+  }
+  if (r == BALE_OK) {
+    int64_t bad = 0;
+    r = bale_sync_errors((void*)s, &bad);
+  } else {
+    (void)hipDeviceSynchronize();
+  }
+  if (r == BALE_OK) {
+    hipEventSynchronize(e3);
+    t_h2d_s = elapsed(e0, e1);
+    t_gpu_s = elapsed(e1, e2);
+    t_d2h_s = elapsed(e2, e3);
+    result = t_gpu_s;
+  }
+  cleanup();
+  return result;
+}
+
+}  // namespace bale
+
+extern "C" {
+
+double histo_agi(int64_t* index, int64_t len, int64_t* counts) {
+  return dropin(0, nullptr, index, len, counts, false, BALE_STRAT_DIRECT, BALE_MERGE_NONE);
+}
+double histo_exstack(int64_t* pckindx, int64_t len, int64_t* lcounts, int64_t) {
+  return dropin(0, nullptr, pckindx, len, lcounts, true, BALE_STRAT_BUCKET, BALE_MERGE_NONE);
+}
+double histo_exstack2(int64_t* pckindx, int64_t len, int64_t* lcounts, int64_t) {
+  return dropin(0, nullptr, pckindx, len, lcounts, true, BALE_STRAT_LDS, BALE_MERGE_NONE);
+}
+double histo_conveyor(int64_t* pckindx, int64_t len, int64_t* lcounts) {
+  return dropin(0, nullptr, pckindx, len, lcounts, true, BALE_STRAT_AUTO, BALE_MERGE_NONE);
+}
+double histo_exstack2_goto(int64_t* pckindx, int64_t T, int64_t* lcounts, int64_t) {
+  return dropin(0, nullptr, pckindx, T, lcounts, true, BALE_STRAT_BUCKET, BALE_MERGE_NONE);
+}
+double histo_exstack2_cyclic(int64_t* pckindx, int64_t T, int64_t* lcounts, int64_t) {
+  return dropin(0, nullptr, pckindx, T, lcounts, true, BALE_STRAT_LDS, BALE_MERGE_RUNS);
+}
+double histo_exstack_function(int64_t* pckindx, int64_t T, int64_t* lcounts, int64_t) {
+  return dropin(0, nullptr, pckindx, T, lcounts, true, BALE_STRAT_BUCKET, BALE_MERGE_NONE);
+}
+double histo_exstack2_function(int64_t* pckindx, int64_t T, int64_t* lcounts, int64_t) {
+  return dropin(0, nullptr, pckindx, T, lcounts, true, BALE_STRAT_LDS, BALE_MERGE_NONE);
+}
+
+double ig_agi(int64_t* tgt, int64_t* index, int64_t l_num_req, int64_t* table) {
+  return dropin(1, tgt, index, l_num_req, table, false, BALE_STRAT_DIRECT, BALE_MERGE_NONE);
+}
+double ig_exstack(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t* ltable, int64_t) {
+  return dropin(1, tgt, pckindx, l_num_req, ltable, true, BALE_STRAT_BUCKET, BALE_MERGE_NONE);
+}
+double ig_exstack2(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t* ltable, int64_t) {
+  return dropin(1, tgt, pckindx, l_num_req, ltable, true, BALE_STRAT_DIRECT, BALE_MERGE_NONE);
+}
+double ig_conveyor(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t* ltable) {
+  return dropin(1, tgt, pckindx, l_num_req, ltable, true, BALE_STRAT_AUTO, BALE_MERGE_NONE);
+}
+double ig_exstack2_cyclic(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t* ltable, int64_t) {
+  return dropin(1, tgt, pckindx, l_num_req, ltable, true, BALE_STRAT_DIRECT, BALE_MERGE_NONE);
+}
+double ig_exstack2_goto(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t* ltable, int64_t) {
+  return dropin(1, tgt, pckindx, l_num_req, ltable, true, BALE_STRAT_DIRECT, BALE_MERGE_NONE);
+}
+double ig_exstack_function(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t* ltable, int64_t) {
+  return dropin(1, tgt, pckindx, l_num_req, ltable, true, BALE_STRAT_BUCKET, BALE_MERGE_NONE);
+}
+double ig_exstack_pkg(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t* ltable, int64_t) {
+  return dropin(1, tgt, pckindx, l_num_req, ltable, true, BALE_STRAT_BUCKET, BALE_MERGE_NONE);
+}
+
+}  // extern "C"

@@ -1,0 +1,385 @@
+// histo.hip -- histogram scatter-add counts[index[i]] += 1 on gfx950.
+//
+// Three strategies (the reference's models, re-thought for one chip):
+//   DIRECT  one device-scope atomic per update, the AGI model
+//           (apps/histo_src/histo_agi.upc:61-66, libgetput/libgetput.upc:48-60).
+//   LDS     per-workgroup sub-histograms of a bin tile in LDS, flushed with one
+//           device atomic per nonzero bin (updates aggregated before they leave
+//           the CU, as exstack2 aggregates before sending).
+//   BUCKET  partition the updates by bin range through LDS send buffers
+//           (partition.h), then one LDS histogram per bucket segment, as
+//           exstack/conveyor route each update to its owner, which then applies
+//           it locally (apps/histo_src/histo_exstack.upc:64-76,
+//           histo_conveyor.upc:72-82).
+// All three add into the existing counts (the drivers accumulate across
+// models, apps/histo_src/histo.upc:208-227).
+#include "bale_internal.h"
+#include "partition.h"
+
+namespace bale {
+
+// ---- wave-level merging of equal bins -------------------------------------
+// key = decoded bin, or ~0 for a lane with nothing to add.  Returns the count
+// this lane must add (0 when another lane adds for it).
+template <int MERGE>
+__device__ inline uint32_t wave_merge(uint64_t key, bool ok) {
+  if (MERGE == 0) return ok ? 1u : 0u;
+  const int lane = __lane_id();
+  uint32_t mine = 0;
+  if (MERGE == 1) {
+    // runs of equal keys in adjacent lanes: the first lane of a run adds its length
+    uint64_t prev = __shfl_up(key, 1);
+    bool change = (lane == 0) || (prev != key);
+    uint64_t chg = __ballot(change);
+    if (ok && change) {
+      uint64_t later = chg & ~((lane == 63) ? ~0ULL : ((2ULL << lane) - 1));
+      int end = later ? __builtin_ctzll(later) : 64;
+      mine = (uint32_t)(end - lane);
+    }
+    return mine;
+  }
+  // MERGE == 2: every distinct key in the wave is added once by a leader lane
+  uint64_t active = __ballot(ok);
+  while (active) {
+    int leader = __builtin_ctzll(active);
+    uint64_t lk = __shfl(key, leader);
+    uint64_t m = __ballot(ok && key == lk) & active;
+    if (lane == leader) mine = (uint32_t)__builtin_popcountll(m);
+    active &= ~m;
+  }
+  return mine;
+}
+
+// ---- DIRECT ---------------------------------------------------------------
+template <int U, typename CT, bool PACKED, int MERGE>
+__global__ __launch_bounds__(1024) void k_histo_direct(const int64_t* __restrict__ idx, int64_t n,
+                                                       CT* __restrict__ counts, uint64_t nbins,
+                                                       ErrWords* err) {
+  const int B = blockDim.x;
+  int64_t base = (int64_t)blockIdx.x * B * U + threadIdx.x;
+  int64_t raw[U];
+#pragma unroll
+  for (int k = 0; k < U; k++) {
+    int64_t e = base + (int64_t)k * B;
+    raw[k] = e < n ? __builtin_nontemporal_load(idx + e) : 0;
+  }
+  unsigned bad_r = 0, bad_p = 0;
+#pragma unroll
+  for (int k = 0; k < U; k++) {
+    int64_t e = base + (int64_t)k * B;
+    uint64_t j = ~0ULL;
+    bool ok = false;
+    if (e < n) ok = decode_index<PACKED>(raw[k], nbins, j, bad_r, bad_p);
+    uint32_t c = wave_merge<MERGE>(ok ? j : ~0ULL, ok);
+    if (c) atomic_add_dev(counts + j, (CT)c);
+  }
+  flush_errors(err, bad_r, bad_p);
+}
+
+// ---- LDS ------------------------------------------------------------------
+// grid = (chunks, tiles); tile t covers bins [t<<tile_bits, +width).
+template <int U, typename CT, bool PACKED, int MERGE>
+__global__ __launch_bounds__(1024) void k_histo_lds(const int64_t* __restrict__ idx, int64_t n,
+                                                    CT* __restrict__ counts, uint64_t nbins,
+                                                    int tile_bits, int64_t chunk, ErrWords* err) {
+  extern __shared__ uint32_t sh[];
+  const int B = blockDim.x;
+  const uint64_t lo_bin = (uint64_t)blockIdx.y << tile_bits;
+  const uint64_t width64 = nbins - lo_bin < (1ULL << tile_bits) ? nbins - lo_bin : (1ULL << tile_bits);
+  const uint32_t width = (uint32_t)width64;
+  for (uint32_t b = threadIdx.x; b < width; b += B) sh[b] = 0;
+  __syncthreads();
+  int64_t lo = (int64_t)blockIdx.x * chunk;
+  int64_t hi = lo + chunk < n ? lo + chunk : n;
+  unsigned bad_r = 0, bad_p = 0;
+  // the loop bound is uniform across the workgroup, so every lane of a wave
+  // reaches wave_merge together
+  for (int64_t base = lo; base < hi; base += (int64_t)B * U) {
+    int64_t raw[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      int64_t e = base + (int64_t)k * B + threadIdx.x;
+      raw[k] = e < hi ? __builtin_nontemporal_load(idx + e) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      int64_t e = base + (int64_t)k * B + threadIdx.x;
+      uint64_t j = ~0ULL;
+      bool ok = false;
+      if (e < hi) ok = decode_index<PACKED>(raw[k], nbins, j, bad_r, bad_p);
+      ok = ok && (j - lo_bin) < width64;
+      uint32_t c = wave_merge<MERGE>(ok ? j : ~0ULL, ok);
+      if (c) atomicAdd(&sh[j - lo_bin], c);
+    }
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < width; b += B) {
+    uint32_t c = sh[b];
+    if (c) atomic_add_dev(counts + lo_bin + b, (CT)c);
+  }
+  // every tile decodes every index; report bad ones once
+  if (blockIdx.y == 0) flush_errors(err, bad_r, bad_p);
+}
+
+// ---- BUCKET ---------------------------------------------------------------
+template <bool PACKED>
+__global__ __launch_bounds__(1024) void k_histo_scatter(const int64_t* __restrict__ idx, int64_t n,
+                                                        uint64_t nbins, int shift, int B, int S,
+                                                        int64_t chunk, uint32_t* __restrict__ cursor,
+                                                        uint16_t* __restrict__ staged) {
+  part_scatter_body<PACKED, uint16_t>(idx, n, nbins, shift, B, S, chunk, cursor, staged,
+                                      [](int64_t, uint64_t, uint64_t low) { return (uint16_t)low; });
+}
+
+// Persistent: each workgroup takes segments (<= seg_len entries of one bucket)
+// from a queue, histograms them in LDS and adds the tile into counts.  A bucket
+// with one segment is owned by one workgroup, so its flush is a plain add.
+template <typename CT>
+__global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict__ staged,
+                                                      const uint32_t* __restrict__ start,
+                                                      const uint32_t* __restrict__ segstart,
+                                                      uint32_t* __restrict__ misc, int B,
+                                                      uint32_t seg_len, int shift, uint64_t nbins,
+                                                      CT* __restrict__ counts) {
+  extern __shared__ uint32_t sh[];
+  __shared__ uint32_t s_work[4];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const uint32_t total = misc[0];
+  while (true) {
+    if (tid == 0) {
+      uint32_t w = atomicAdd(&misc[1], 1u);
+      uint32_t b = 0;
+      if (w < total) {
+        // last bucket whose first segment is <= w
+        int lo = 0, hi = B - 1;
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (segstart[mid] <= w) lo = mid; else hi = mid - 1;
+        }
+        b = (uint32_t)lo;
+      }
+      s_work[0] = w;
+      s_work[1] = b;
+    }
+    __syncthreads();
+    const uint32_t w = s_work[0], b = s_work[1];
+    if (w >= total) break;
+    const uint32_t nseg = segstart[b + 1] - segstart[b];
+    const uint32_t s = w - segstart[b];
+    const uint32_t e0 = start[b] + s * seg_len;
+    const uint32_t e1 = min(start[b + 1], e0 + seg_len);
+    const uint64_t lo_bin = (uint64_t)b << shift;
+    const uint32_t width = (uint32_t)min((uint64_t)1 << shift, nbins - lo_bin);
+    for (uint32_t k = tid; k < width; k += nthr) sh[k] = 0;
+    __syncthreads();
+    // unaligned head, then 8 entries per 16-byte load
+    uint32_t a0 = min(e1, (e0 + 7u) & ~7u);
+    for (uint32_t k = e0 + tid; k < a0; k += nthr) atomicAdd(&sh[staged[k]], 1u);
+    uint32_t nvec = (e1 - a0) / 8u;
+    const u32x4* v = (const u32x4*)(staged + a0);
+    for (uint32_t q = tid; q < nvec; q += nthr) {
+      u32x4 x = __builtin_nontemporal_load(v + q);
+      atomicAdd(&sh[x.x & 0xffff], 1u);
+      atomicAdd(&sh[x.x >> 16], 1u);
+      atomicAdd(&sh[x.y & 0xffff], 1u);
+      atomicAdd(&sh[x.y >> 16], 1u);
+      atomicAdd(&sh[x.z & 0xffff], 1u);
+      atomicAdd(&sh[x.z >> 16], 1u);
+      atomicAdd(&sh[x.w & 0xffff], 1u);
+      atomicAdd(&sh[x.w >> 16], 1u);
+    }
+    for (uint32_t k = a0 + nvec * 8u + tid; k < e1; k += nthr) atomicAdd(&sh[staged[k]], 1u);
+    __syncthreads();
+    CT* dst = counts + lo_bin;
+    if (nseg == 1) {
+      for (uint32_t k = tid; k < width; k += nthr) {
+        uint32_t c = sh[k];
+        if (c) dst[k] += (CT)c;
+      }
+    } else {
+      for (uint32_t k = tid; k < width; k += nthr) {
+        uint32_t c = sh[k];
+        if (c) atomic_add_dev(dst + k, (CT)c);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- launchers ------------------------------------------------------------
+template <int U, typename CT, bool PACKED>
+static void go_direct(int merge, dim3 g, dim3 b, hipStream_t s, const int64_t* idx, int64_t n,
+                      CT* counts, uint64_t nbins, ErrWords* err) {
+  if (merge == 1)
+    hipLaunchKernelGGL((k_histo_direct<U, CT, PACKED, 1>), g, b, 0, s, idx, n, counts, nbins, err);
+  else if (merge == 2)
+    hipLaunchKernelGGL((k_histo_direct<U, CT, PACKED, 2>), g, b, 0, s, idx, n, counts, nbins, err);
+  else
+    hipLaunchKernelGGL((k_histo_direct<U, CT, PACKED, 0>), g, b, 0, s, idx, n, counts, nbins, err);
+}
+
+template <typename CT, bool PACKED>
+static void go_direct_u(int U, int merge, dim3 g, dim3 b, hipStream_t s, const int64_t* idx,
+                        int64_t n, CT* counts, uint64_t nbins, ErrWords* err) {
+  if (U == 16) go_direct<16, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, err);
+  else if (U == 8) go_direct<8, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, err);
+  else go_direct<4, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, err);
+}
+
+static int norm_block(int b) { return (b == 256 || b == 512 || b == 1024) ? b : 256; }
+static int norm_unroll(int u) { return (u == 4 || u == 8 || u == 16) ? u : 8; }
+
+int launch_histo_direct(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
+                        bool packed, const bale_config& c, ErrWords* err, hipStream_t s) {
+  if (n <= 0) return 0;
+  int block = norm_block(c.block), U = norm_unroll(c.unroll);
+  int merge = c.merge < 0 ? 0 : c.merge;
+  int64_t per = (int64_t)block * U;
+  int64_t grid = ceil_div(n, per);
+  if (grid > 0x7fffffff) return (int)hipErrorInvalidValue;
+  dim3 g((unsigned)grid), b(block);
+  if (dtype == BALE_I64) {
+    if (packed) go_direct_u<int64_t, true>(U, merge, g, b, s, idx, n, (int64_t*)counts, nbins, err);
+    else go_direct_u<int64_t, false>(U, merge, g, b, s, idx, n, (int64_t*)counts, nbins, err);
+  } else {
+    if (packed) go_direct_u<int32_t, true>(U, merge, g, b, s, idx, n, (int32_t*)counts, nbins, err);
+    else go_direct_u<int32_t, false>(U, merge, g, b, s, idx, n, (int32_t*)counts, nbins, err);
+  }
+  return (int)hipGetLastError();
+}
+
+template <int U, typename CT, bool PACKED>
+static void go_lds(int merge, dim3 g, dim3 b, size_t lds, hipStream_t s, const int64_t* idx,
+                   int64_t n, CT* counts, uint64_t nbins, int tb, int64_t chunk, ErrWords* err) {
+  if (merge == 1) {
+    allow_lds(k_histo_lds<U, CT, PACKED, 1>, lds);
+    hipLaunchKernelGGL((k_histo_lds<U, CT, PACKED, 1>), g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
+  } else if (merge == 2) {
+    allow_lds(k_histo_lds<U, CT, PACKED, 2>, lds);
+    hipLaunchKernelGGL((k_histo_lds<U, CT, PACKED, 2>), g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
+  } else {
+    allow_lds(k_histo_lds<U, CT, PACKED, 0>, lds);
+    hipLaunchKernelGGL((k_histo_lds<U, CT, PACKED, 0>), g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
+  }
+}
+
+template <typename CT, bool PACKED>
+static void go_lds_u(int U, int merge, dim3 g, dim3 b, size_t lds, hipStream_t s,
+                     const int64_t* idx, int64_t n, CT* counts, uint64_t nbins, int tb,
+                     int64_t chunk, ErrWords* err) {
+  if (U == 16) go_lds<16, CT, PACKED>(merge, g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
+  else if (U == 8) go_lds<8, CT, PACKED>(merge, g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
+  else go_lds<4, CT, PACKED>(merge, g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
+}
+
+// Largest bin tile (log2) whose 32-bit LDS counters leave 1 KiB of the
+// per-workgroup LDS for static variables.
+static int max_tile_bits(const Device* d) {
+  int tb = 15;
+  while (tb > 8 && ((size_t)4 << tb) + 1024 > (size_t)d->lds_per_block) tb--;
+  return tb;
+}
+
+int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
+                     bool packed, const bale_config& c, Device* d, hipStream_t s) {
+  if (n <= 0) return 0;
+  int block = norm_block(c.block), U = norm_unroll(c.unroll);
+  int merge = c.merge < 0 ? 0 : c.merge;
+  int tb = c.tile_log2 > 0 ? c.tile_log2 : max_tile_bits(d);
+  if (tb > max_tile_bits(d)) tb = max_tile_bits(d);
+  int64_t width = std::min<int64_t>((int64_t)1 << tb, nbins);
+  while (tb > 8 && ((int64_t)1 << (tb - 1)) >= width) tb--;  // shrink to the table
+  int64_t tiles = ceil_div(nbins, (int64_t)1 << tb);
+  if (tiles > 65535) return (int)hipErrorInvalidValue;
+  int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 2;
+  int64_t target = (int64_t)d->cus * gpc;
+  int64_t chunks = std::max<int64_t>(1, target / tiles);
+  // keep the flush (one atomic per bin per chunk) well under the update count
+  int64_t cap = std::max<int64_t>(1, n / (4 * std::min<int64_t>(nbins, (int64_t)1 << tb)));
+  chunks = std::min(chunks, cap);
+  int64_t chunk = ceil_div(n, chunks);
+  chunks = ceil_div(n, chunk);
+  dim3 g((unsigned)chunks, (unsigned)tiles), b(block);
+  size_t lds = (size_t)4 << tb;
+  if (dtype == BALE_I64) {
+    if (packed) go_lds_u<int64_t, true>(U, merge, g, b, lds, s, idx, n, (int64_t*)counts, nbins, tb, chunk, d->err);
+    else go_lds_u<int64_t, false>(U, merge, g, b, lds, s, idx, n, (int64_t*)counts, nbins, tb, chunk, d->err);
+  } else {
+    if (packed) go_lds_u<int32_t, true>(U, merge, g, b, lds, s, idx, n, (int32_t*)counts, nbins, tb, chunk, d->err);
+    else go_lds_u<int32_t, false>(U, merge, g, b, lds, s, idx, n, (int32_t*)counts, nbins, tb, chunk, d->err);
+  }
+  return (int)hipGetLastError();
+}
+
+constexpr int kMaxBuckets = 4096;
+
+int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
+                        bool packed, const bale_config& c, Device* d, hipStream_t s) {
+  if (n <= 0) return 0;
+  if (n >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;  // caller splits
+  int tb = c.tile_log2 > 0 ? c.tile_log2 : max_tile_bits(d);
+  if (tb > max_tile_bits(d)) tb = max_tile_bits(d);
+  if (tb > 16) tb = 16;  // entries are 16-bit bin offsets
+  int64_t B = ceil_div(nbins, (int64_t)1 << tb);
+  if (B > kMaxBuckets) return (int)hipErrorInvalidValue;
+  size_t bytes = part_bytes(B, n, sizeof(uint16_t));
+  void* ws = workspace(d, bytes, s);
+  if (!ws) return (int)hipErrorOutOfMemory;
+  PartLayout L = part_carve(ws, B);
+  int rc = launch_zero_bytes(ws, part_zero_bytes(B), s);
+  if (rc) return rc;
+  int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
+  // pass 1: count
+  {
+    int64_t groups = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * 4, ceil_div(n, 4096)));
+    int64_t chunk = ceil_div(n, groups);
+    groups = ceil_div(n, chunk);
+    size_t lds = (size_t)B * 4;
+    if (packed)
+      hipLaunchKernelGGL(k_part_count<true>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
+                         (uint64_t)nbins, tb, (int)B, chunk, L.count, d->err);
+    else
+      hipLaunchKernelGGL(k_part_count<false>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
+                         (uint64_t)nbins, tb, (int)B, chunk, L.count, d->err);
+  }
+  // segments: enough of them to spread the apply pass over the chip
+  int64_t seg = std::max<int64_t>(4096, ceil_div(n, (int64_t)d->cus * 2));
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.count, (int)B, (uint32_t)seg,
+                     L.start, L.cursor, L.segstart, L.misc);
+  // pass 3: scatter through LDS send buffers
+  {
+    size_t budget = (size_t)d->lds_per_block;
+    int S = part_pick_depth((int)B, sizeof(uint16_t), budget);
+    if (S == 0) return (int)hipErrorInvalidValue;
+    size_t lds = part_scatter_lds((int)B, S, sizeof(uint16_t));
+    allow_lds(k_histo_scatter<true>, lds);
+    allow_lds(k_histo_scatter<false>, lds);
+    int64_t groups = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 1024)));
+    int64_t chunk = ceil_div(n, groups);
+    groups = ceil_div(n, chunk);
+    if (packed)
+      hipLaunchKernelGGL(k_histo_scatter<true>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
+                         (uint64_t)nbins, tb, (int)B, S, chunk, L.cursor, (uint16_t*)L.staged);
+    else
+      hipLaunchKernelGGL(k_histo_scatter<false>, dim3((unsigned)groups), dim3(1024), lds, s, idx,
+                         n, (uint64_t)nbins, tb, (int)B, S, chunk, L.cursor, (uint16_t*)L.staged);
+  }
+  // apply
+  {
+    size_t lds = (size_t)4 << tb;
+    int64_t groups = (int64_t)d->cus * (lds <= 65536 ? 2 : 1);
+    allow_lds(k_histo_apply<int64_t>, lds);
+    allow_lds(k_histo_apply<int32_t>, lds);
+    if (dtype == BALE_I64)
+      hipLaunchKernelGGL(k_histo_apply<int64_t>, dim3((unsigned)groups), dim3(1024), lds, s,
+                         (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B,
+                         (uint32_t)seg, tb, (uint64_t)nbins, (int64_t*)counts);
+    else
+      hipLaunchKernelGGL(k_histo_apply<int32_t>, dim3((unsigned)groups), dim3(1024), lds, s,
+                         (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B,
+                         (uint32_t)seg, tb, (uint64_t)nbins, (int32_t*)counts);
+  }
+  return (int)hipGetLastError();
+}
+
+}  // namespace bale

@@ -1,0 +1,161 @@
+/*
+ * bale_hip.h -- C-ABI of libbale_hip.so, the MI355X (gfx950) implementation
+ * of bale's histogram scatter-add and index gather.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-ins with the reference's exact prototypes (SHARED expands to
+ *     nothing, as in the reference's SHMEM build, libgetput/libgetput.h:151).
+ *     They run on one GPU, which is one PE (THREADS == 1), so a packed
+ *     index is (lindx << 16) | 0.  Pointers may be host or device memory:
+ *     host arrays are staged to the GPU and back (histograms are added to,
+ *     never overwritten; gathers overwrite tgt[0..n)).  Each returns the GPU
+ *     seconds of the operation with data resident (HIP events), or -1.0 on
+ *     error (bale_hip_last_error() says why), as the reference returns -1.0
+ *     when setup fails (apps/histo_src/histo_exstack.upc:58).
+ *
+ *  2. The native API (bale_*): device pointers, explicit sizes, counter
+ *     type, strategy and tile shape, asynchronous on a caller's HIP stream.
+ *
+ * Indices are bounds-checked on the GPU: an index outside the table is
+ * skipped and counted, and the call reports BALE_ERR_RANGE (the reference
+ * does no check; an out-of-range index there is undefined behaviour).
+ */
+#ifndef BALE_HIP_H
+#define BALE_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================= 1. reference drop-ins ======================== */
+
+/* apps/histo_src/histo.h:47 -- AGI model: counts[index[i]] += 1.
+ * Runs the direct-atomic strategy. */
+double histo_agi(int64_t *index, int64_t len, int64_t *counts);
+/* apps/histo_src/histo.h:48 -- exstack model: packed indices, bulk-synchronous
+ * aggregation.  Runs the bucket-by-destination strategy (partition by bin
+ * range, then LDS histogram per bucket).  buf_cnt is accepted and ignored. */
+double histo_exstack(int64_t *pckindx, int64_t len, int64_t *lcounts, int64_t buf_cnt);
+/* apps/histo_src/histo.h:49 -- exstack2 model: runs the LDS-privatized
+ * strategy (one pass, per-workgroup sub-histograms). */
+double histo_exstack2(int64_t *pckindx, int64_t len, int64_t *lcounts, int64_t buf_cnt);
+/* apps/histo_src/histo.h:50 -- conveyor model: the library picks the
+ * strategy (autotuner cache, else its default rule). */
+double histo_conveyor(int64_t *pckindx, int64_t len, int64_t *lcounts);
+/* apps/histo_src/alternates/histo_alternates.h:43-46 -- same results; the
+ * goto/cyclic/function variants differ only in control flow in the
+ * reference.  They map to: goto -> bucket, cyclic -> LDS with adjacent-run
+ * merging, exstack_function -> bucket, exstack2_function -> LDS. */
+double histo_exstack2_goto(int64_t *pckindx, int64_t T, int64_t *lcounts, int64_t buf_cnt);
+double histo_exstack2_cyclic(int64_t *pckindx, int64_t T, int64_t *lcounts, int64_t buf_cnt);
+double histo_exstack_function(int64_t *pckindx, int64_t T, int64_t *lcounts, int64_t buf_cnt);
+double histo_exstack2_function(int64_t *pckindx, int64_t T, int64_t *lcounts, int64_t buf_cnt);
+
+/* apps/ig_src/ig.h:49 -- AGI model: tgt[i] = table[index[i]] (direct gather). */
+double ig_agi(int64_t *tgt, int64_t *index, int64_t l_num_req, int64_t *table);
+/* apps/ig_src/ig.h:50 -- exstack model: packed indices; runs the bucketed
+ * gather (requests grouped by table region, replies written to their slot). */
+double ig_exstack(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable, int64_t buf_cnt);
+/* apps/ig_src/ig.h:51 -- exstack2 model: packed indices, direct gather. */
+double ig_exstack2(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable, int64_t buf_cnt);
+/* apps/ig_src/ig.h:52 -- conveyor model: library-picked strategy. */
+double ig_conveyor(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable);
+/* apps/ig_src/alternates/ig_alternates.h:42-45 -- same results:
+ * exstack2_cyclic / exstack2_goto -> direct, exstack_function / exstack_pkg
+ * -> bucketed. */
+double ig_exstack2_cyclic(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable, int64_t buf_cnt);
+double ig_exstack2_goto(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable, int64_t buf_cnt);
+double ig_exstack_function(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable, int64_t buf_cnt);
+double ig_exstack_pkg(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable, int64_t buf_cnt);
+
+/* ============================ 2. native API =========================== */
+
+#define BALE_OK          0
+#define BALE_ERR_ARG    -1  /* bad argument (null pointer, size, enum) */
+#define BALE_ERR_RANGE  -2  /* some indices were outside [0, table_len): skipped */
+#define BALE_ERR_HIP    -3  /* a HIP runtime call failed */
+#define BALE_ERR_ALLOC  -4  /* device memory could not be allocated */
+#define BALE_ERR_PE     -5  /* a packed index names a PE other than 0 */
+#define BALE_ERR_NODEV  -6  /* no usable GPU */
+
+/* strategies */
+#define BALE_STRAT_AUTO    0
+#define BALE_STRAT_DIRECT  1  /* histogram: one device atomic per update; gather: one load per request */
+#define BALE_STRAT_LDS     2  /* histogram: LDS sub-histograms per workgroup, bin-range tiled */
+#define BALE_STRAT_BUCKET  3  /* partition by destination range in LDS, coalesced flush, then apply */
+
+/* counter types */
+#define BALE_I64 0
+#define BALE_I32 1
+
+/* index formats */
+#define BALE_INDEX_PLAIN  0  /* global index */
+#define BALE_INDEX_PACKED 1  /* (lindx << 16) | pe, pe must be 0 */
+
+/* wave-level merging of duplicate bins before the add */
+#define BALE_MERGE_NONE   0
+#define BALE_MERGE_RUNS   1  /* adjacent lanes with equal bins are combined */
+#define BALE_MERGE_LEADER 2  /* plus up to 4 rounds of leader-key ballot merging */
+
+typedef struct bale_config {
+  int32_t strategy;   /* BALE_STRAT_* */
+  int32_t block;      /* threads per workgroup: 256, 512 or 1024; 0 = default */
+  int32_t unroll;     /* items per lane per step: 4, 8 or 16; 0 = default */
+  int32_t tile_log2;  /* LDS bin tile / bucket width in log2 bins; 0 = default */
+  int32_t merge;      /* BALE_MERGE_*; -1 = default */
+  int32_t groups_per_cu; /* resident workgroups per CU for persistent passes; 0 = default */
+} bale_config;
+
+/* Select the device for this thread's later calls and create its state.
+ * Optional: the first call on a device does it. */
+int bale_hip_init(int device);
+/* Text of the last error on this thread ("" if none). */
+const char *bale_hip_last_error(void);
+/* GPU seconds, host-to-device and device-to-host seconds of the last
+ * drop-in call on this thread. */
+void bale_hip_last_timing(double *gpu_s, double *h2d_s, double *d2h_s);
+
+/* counts[index[i]] += 1 for i < n (counts is int64_t or int32_t per dtype).
+ * Asynchronous on `stream` (a hipStream_t; NULL = the null stream).  Range
+ * and PE errors are accumulated on the device; read them with
+ * bale_sync_errors(). */
+int bale_histo(const int64_t *index, int64_t n, void *counts, int64_t nbins,
+               int dtype, int index_format, const bale_config *cfg, void *stream);
+/* tgt[i] = table[index[i]] for i < n. */
+int bale_gather(int64_t *tgt, const int64_t *index, int64_t n, const int64_t *table,
+                int64_t table_len, int index_format, const bale_config *cfg, void *stream);
+/* Wait for `stream`, then return BALE_OK, BALE_ERR_RANGE or BALE_ERR_PE for the
+ * calls made since the last check on this device, and clear the record.
+ * *bad (optional) receives the number of skipped indices. */
+int bale_sync_errors(void *stream, int64_t *bad);
+/* The configuration BALE_STRAT_AUTO would run for this shape. */
+int bale_histo_plan(int64_t n, int64_t nbins, int dtype, bale_config *out);
+int bale_gather_plan(int64_t n, int64_t table_len, bale_config *out);
+
+/* ---- input generation on the GPU (seeded, reproducible) ------------- */
+/* table[j] = splitmix64(j) */
+int bale_fill_splitmix(int64_t *table, int64_t n, void *stream);
+/* table[j] = -(j+1) (the reference ig driver's fill, apps/ig_src/ig.upc:161) */
+int bale_fill_neg(int64_t *table, int64_t n, void *stream);
+/* Index streams: dist 0 uniform over [0,nbins); 1 Zipf(s) ranks over a keyed
+ * permutation of the bins; run_mean > 1 makes runs of a repeated value with
+ * geometric lengths of that mean.  Element i depends only on (seed, i). */
+int bale_gen_index(int64_t *out, int64_t n, int64_t nbins, int dist, double s,
+                   int64_t run_mean, uint64_t seed, void *stream);
+/* pck[i] = index[i] << 16: the packed form at one PE (apps/histo_src/histo.upc:148-150). */
+int bale_pack(int64_t *pck, const int64_t *index, int64_t n, void *stream);
+
+/* ---- roofline probes ------------------------------------------------- */
+/* dst[i] = src[i] over `bytes` (multiple of 16): streaming copy rate. */
+int bale_probe_copy(void *dst, const void *src, int64_t bytes, void *stream);
+/* nreads random 8-byte reads of table[0..n) at addresses hashed from the
+ * read number (no index stream); a digest goes to *out (one int64). */
+int bale_probe_random_read(const int64_t *table, int64_t n, int64_t nreads,
+                           int64_t *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,348 @@
+/*
+ * bale_oracle.c -- CPU restatement of bale's histogram / index-gather hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * Reference citations are file:line in the jdevinney/bale_old tree.
+ */
+#define _GNU_SOURCE
+#include "bale_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#define GOLDEN 0x9E3779B97F4A7C15ULL
+
+static inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t or_splitmix64_next(uint64_t *state) {
+  *state += GOLDEN;
+  return mix64(*state);
+}
+
+uint64_t or_splitmix64_at(uint64_t seed, uint64_t i) {
+  return mix64(seed + (i + 1) * GOLDEN);
+}
+
+uint64_t or_splitmix64(uint64_t j) { return mix64(j + GOLDEN); }
+
+uint64_t or_reduce(uint64_t x, uint64_t n) {
+  if (n <= 1) return 0;
+  if ((n & (n - 1)) == 0) {
+    int k = __builtin_ctzll(n);
+    return x >> (64 - k);
+  }
+  return (uint64_t)(((unsigned __int128)x * n) >> 64);
+}
+
+/* ---- keyed permutation ------------------------------------------------ */
+uint64_t or_perm(uint64_t x, uint64_t n, uint64_t key) {
+  if (n <= 1) return 0;
+  int bits = 64 - __builtin_clzll(n - 1); /* smallest b with 2^b >= n */
+  int half = (bits + 1) / 2;
+  if (half < 1) half = 1;
+  uint64_t mask = (1ULL << half) - 1;
+  do {
+    uint64_t L = x >> half, R = x & mask;
+    for (int r = 0; r < 4; r++) {
+      uint64_t f = mix64(R ^ or_splitmix64_at(key, (uint64_t)r)) & mask;
+      uint64_t t = R;
+      R = L ^ f;
+      L = t;
+    }
+    x = (L << half) | R;
+  } while (x >= n);
+  return x;
+}
+
+/* ---- Zipf by rejection-inversion -------------------------------------- */
+/* h(x) = x^-s, H(x) = (x^(1-s) - 1)/(1-s) written with expm1/log1p so that
+ * s -> 1 stays finite. */
+typedef struct zipf_t {
+  double s, hx1, hn, sc;
+  int64_t n;
+} zipf_t;
+
+static double z_helper1(double t) { return fabs(t) > 1e-8 ? log1p(t) / t : 1.0 - t * (0.5 - t / 3.0); }
+static double z_helper2(double t) { return fabs(t) > 1e-8 ? expm1(t) / t : 1.0 + t * 0.5 * (1.0 + t / 3.0); }
+static double z_H(const zipf_t *z, double x) {
+  double lx = log(x);
+  return z_helper2((1.0 - z->s) * lx) * lx;
+}
+static double z_h(const zipf_t *z, double x) { return exp(-z->s * log(x)); }
+static double z_Hinv(const zipf_t *z, double y) {
+  double t = y * (1.0 - z->s);
+  if (t < -1.0) t = -1.0;
+  return exp(z_helper1(t) * y);
+}
+static void zipf_init(zipf_t *z, int64_t n, double s) {
+  z->n = n;
+  z->s = s;
+  z->hx1 = z_H(z, 1.5) - 1.0;
+  z->hn = z_H(z, (double)n + 0.5);
+  z->sc = 2.0 - z_Hinv(z, z_H(z, 2.5) - z_h(z, 2.0));
+}
+static int64_t zipf_draw(const zipf_t *z, uint64_t subseed) {
+  for (uint64_t k = 0;; k++) {
+    uint64_t r = or_splitmix64_at(subseed, k);
+    double u01 = (double)(r >> 11) * 0x1.0p-53;
+    double u = z->hn + u01 * (z->hx1 - z->hn);
+    double x = z_Hinv(z, u);
+    int64_t kk = (int64_t)(x + 0.5);
+    if (kk < 1) kk = 1;
+    if (kk > z->n) kk = z->n;
+    if ((double)kk - x <= z->sc || u >= z_H(z, (double)kk + 0.5) - z_h(z, (double)kk))
+      return kk;
+  }
+}
+
+double or_zipf_pmf(int64_t k, int64_t n, double s) {
+  double norm = 0.0;
+  for (int64_t j = n; j >= 1; j--) norm += pow((double)j, -s);
+  return pow((double)k, -s) / norm;
+}
+
+#define RUN_CAP 4096
+#define PERM_SALT 0x5045524D5045524DULL /* "PERMPERM" */
+#define FLAG_SALT 0x52554E5352554E53ULL /* "RUNSRUNS" */
+
+static int64_t draw_value(int64_t i, int64_t nbins, int dist, const zipf_t *z,
+                          uint64_t seed, uint64_t pkey) {
+  uint64_t r = or_splitmix64_at(seed, (uint64_t)i);
+  if (dist == 0) return (int64_t)or_reduce(r, (uint64_t)nbins);
+  int64_t rank = zipf_draw(z, r);
+  return (int64_t)or_perm((uint64_t)(rank - 1), (uint64_t)nbins, pkey);
+}
+
+static int is_run_start(int64_t i, int64_t run_mean, uint64_t seed) {
+  if (i % RUN_CAP == 0) return 1;
+  uint64_t f = or_splitmix64_at(seed ^ FLAG_SALT, (uint64_t)i);
+  return or_reduce(f, (uint64_t)run_mean) == 0;
+}
+
+void or_gen_index(int64_t *out, int64_t n, int64_t nbins, int dist, double s,
+                  int64_t run_mean, uint64_t seed) {
+  zipf_t z;
+  if (dist == 1) zipf_init(&z, nbins, s);
+  uint64_t pkey = mix64(seed ^ PERM_SALT);
+  int64_t cur = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (run_mean > 1) {
+      if (is_run_start(i, run_mean, seed)) cur = draw_value(i, nbins, dist, &z, seed, pkey);
+      out[i] = cur;
+    } else {
+      out[i] = draw_value(i, nbins, dist, &z, seed, pkey);
+    }
+  }
+}
+
+void or_gen_uniform(int64_t *out, int64_t n, int64_t range, uint64_t seed) {
+  for (int64_t i = 0; i < n; i++)
+    out[i] = (int64_t)or_reduce(or_splitmix64_at(seed, (uint64_t)i), (uint64_t)range);
+}
+
+void or_fill_splitmix(int64_t *table, int64_t n) {
+  for (int64_t j = 0; j < n; j++) table[j] = (int64_t)or_splitmix64((uint64_t)j);
+}
+
+void or_fill_neg(int64_t *table, int64_t n) {
+  for (int64_t j = 0; j < n; j++) table[j] = -(j + 1);
+}
+
+void or_pack(int64_t *pck, const int64_t *idx, int64_t n, int64_t P) {
+  for (int64_t i = 0; i < n; i++) {
+    int64_t lindx = idx[i] / P, pe = idx[i] % P;
+    pck[i] = (lindx << 16) | (pe & 0xffff);
+  }
+}
+
+void or_gen_reference_rand(int64_t *out, int64_t n, int64_t range, unsigned seed0) {
+  srand(seed0);
+  for (int64_t i = 0; i < n; i++) out[i] = rand() % range;
+}
+
+/* ---- histogram --------------------------------------------------------- */
+void or_histo_agi(const int64_t *index, int64_t n, int64_t *counts) {
+  for (int64_t i = 0; i < n; i++) counts[index[i]] += 1;
+}
+
+void or_histo_agi_i32(const int64_t *index, int64_t n, int32_t *counts) {
+  for (int64_t i = 0; i < n; i++) counts[index[i]] += 1;
+}
+
+int or_histo_owner(const int64_t *pckindx, int64_t n, int64_t *lcounts,
+                   int64_t lnum, int64_t P) {
+  /* push phase: route each decoded column to its owner's queue, in order */
+  int64_t *cnt = calloc((size_t)P + 1, sizeof(int64_t));
+  for (int64_t i = 0; i < n; i++) {
+    int64_t pe = pckindx[i] & 0xffff;
+    if (pe >= P) { free(cnt); return -1; }
+    cnt[pe + 1]++;
+  }
+  for (int64_t p = 0; p < P; p++) cnt[p + 1] += cnt[p];
+  int64_t *q = malloc((size_t)(n ? n : 1) * sizeof(int64_t));
+  int64_t *pos = malloc((size_t)P * sizeof(int64_t));
+  memcpy(pos, cnt, (size_t)P * sizeof(int64_t));
+  for (int64_t i = 0; i < n; i++) {
+    int64_t pe = pckindx[i] & 0xffff;
+    q[pos[pe]++] = pckindx[i] >> 16;
+  }
+  /* pull phase: every PE applies its queue to its local slice */
+  for (int64_t p = 0; p < P; p++)
+    for (int64_t k = cnt[p]; k < cnt[p + 1]; k++) lcounts[p * lnum + q[k]] += 1;
+  free(q); free(pos); free(cnt);
+  return 0;
+}
+
+/* ---- index gather ------------------------------------------------------ */
+void or_ig_agi(int64_t *tgt, const int64_t *index, int64_t n, const int64_t *table) {
+  for (int64_t i = 0; i < n; i++) tgt[i] = table[index[i]];
+}
+
+int or_ig_exstack(int64_t *tgt, const int64_t *pckindx, int64_t n,
+                  const int64_t *ltables, int64_t lnum, int64_t P) {
+  int64_t *cnt = calloc((size_t)P + 1, sizeof(int64_t));
+  for (int64_t i = 0; i < n; i++) {
+    int64_t pe = pckindx[i] & 0xffff;
+    if (pe >= P) { free(cnt); return -1; }
+    cnt[pe + 1]++;
+  }
+  for (int64_t p = 0; p < P; p++) cnt[p + 1] += cnt[p];
+  int64_t *q = malloc((size_t)(n ? n : 1) * sizeof(int64_t));
+  int64_t *pos = malloc((size_t)P * sizeof(int64_t));
+  memcpy(pos, cnt, (size_t)P * sizeof(int64_t));
+  /* requests: raw local indices, FIFO per destination */
+  for (int64_t i = 0; i < n; i++) {
+    int64_t pe = pckindx[i] & 0xffff;
+    q[pos[pe]++] = pckindx[i] >> 16;
+  }
+  /* owners replace each request in place by its table value (the reply) */
+  for (int64_t p = 0; p < P; p++)
+    for (int64_t k = cnt[p]; k < cnt[p + 1]; k++) q[k] = ltables[p * lnum + q[k]];
+  /* retrace the requests in order, popping each source's FIFO */
+  memcpy(pos, cnt, (size_t)P * sizeof(int64_t));
+  for (int64_t j = 0; j < n; j++) {
+    int64_t from = pckindx[j] & 0xffff;
+    tgt[j] = q[pos[from]++];
+  }
+  free(q); free(pos); free(cnt);
+  return 0;
+}
+
+int or_ig_conveyor(int64_t *tgt, const int64_t *pckindx, int64_t n,
+                   const int64_t *ltables, int64_t lnum, int64_t P) {
+  /* the reply carries its request slot, so delivery order is irrelevant:
+   * serve the owners in reverse to make that visible */
+  for (int64_t p = P - 1; p >= 0; p--)
+    for (int64_t i = n - 1; i >= 0; i--) {
+      int64_t pe = pckindx[i] & 0xffff;
+      if (pe >= P) return -1;
+      if (pe != p) continue;
+      int64_t val = ltables[p * lnum + (pckindx[i] >> 16)];
+      tgt[i] = val;
+    }
+  return 0;
+}
+
+/* ---- timed multi-thread CPU baseline ---------------------------------- */
+typedef struct mt_arg {
+  const int64_t *index;
+  int64_t lo, hi;
+  int64_t *counts;
+  int64_t *tgt;
+  const int64_t *table;
+  pthread_barrier_t *bar;
+  double secs;
+  int gather;
+} mt_arg;
+
+static double wall_seconds(void) {
+  struct timeval tp;
+  gettimeofday(&tp, NULL);
+  return (double)tp.tv_sec + (double)tp.tv_usec * 1.e-6;
+}
+
+static void *mt_body(void *p) {
+  mt_arg *a = (mt_arg *)p;
+  pthread_barrier_wait(a->bar);
+  double t0 = wall_seconds();
+  if (a->gather) {
+    for (int64_t i = a->lo; i < a->hi; i++) a->tgt[i] = a->table[a->index[i]];
+  } else {
+    for (int64_t i = a->lo; i < a->hi; i++)
+      __atomic_fetch_add(&a->counts[a->index[i]], 1, __ATOMIC_RELAXED);
+  }
+  pthread_barrier_wait(a->bar);
+  a->secs = wall_seconds() - t0;
+  return NULL;
+}
+
+static double run_mt(const int64_t *index, int64_t n, int64_t *counts, int64_t *tgt,
+                     const int64_t *table, int nthreads, int gather) {
+  if (nthreads < 1) nthreads = 1;
+  pthread_t *th = malloc(sizeof(pthread_t) * (size_t)nthreads);
+  mt_arg *args = malloc(sizeof(mt_arg) * (size_t)nthreads);
+  pthread_barrier_t bar;
+  pthread_barrier_init(&bar, NULL, (unsigned)nthreads);
+  for (int t = 0; t < nthreads; t++) {
+    args[t].index = index;
+    args[t].lo = n * t / nthreads;
+    args[t].hi = n * (t + 1) / nthreads;
+    args[t].counts = counts;
+    args[t].tgt = tgt;
+    args[t].table = table;
+    args[t].bar = &bar;
+    args[t].gather = gather;
+    pthread_create(&th[t], NULL, mt_body, &args[t]);
+  }
+  double sum = 0.0;
+  for (int t = 0; t < nthreads; t++) {
+    pthread_join(th[t], NULL);
+    sum += args[t].secs;
+  }
+  pthread_barrier_destroy(&bar);
+  free(th); free(args);
+  return sum / nthreads;
+}
+
+double or_histo_agi_mt(const int64_t *index, int64_t n, int64_t *counts, int nthreads) {
+  return run_mt(index, n, counts, NULL, NULL, nthreads, 0);
+}
+
+double or_ig_agi_mt(int64_t *tgt, const int64_t *index, int64_t n,
+                    const int64_t *table, int nthreads) {
+  return run_mt(index, n, NULL, tgt, table, nthreads, 1);
+}
+
+/* ---- batched mix ------------------------------------------------------- */
+void or_batch_params(or_instance *out, int64_t count, uint64_t seed,
+                     int lo_tab, int hi_tab, int lo_ops, int hi_ops) {
+  for (int64_t t = 0; t < count; t++) {
+    uint64_t sub = or_splitmix64_at(seed, (uint64_t)t);
+    uint64_t d0 = or_splitmix64_at(sub, 0), d1 = or_splitmix64_at(sub, 1);
+    uint64_t d2 = or_splitmix64_at(sub, 2), d3 = or_splitmix64_at(sub, 3);
+    uint64_t d4 = or_splitmix64_at(sub, 4), d5 = or_splitmix64_at(sub, 5);
+    double u1 = (double)(d1 >> 11) * 0x1.0p-53, u2 = (double)(d2 >> 11) * 0x1.0p-53;
+    double u4 = (double)(d4 >> 11) * 0x1.0p-53;
+    or_instance *I = &out[t];
+    I->kind = (int32_t)(d0 >> 63);
+    I->table_len = (int64_t)floor(exp2((double)lo_tab + (double)(hi_tab - lo_tab) * u1));
+    I->nops = (int64_t)floor(exp2((double)lo_ops + (double)(hi_ops - lo_ops) * u2));
+    I->dist = (int32_t)or_reduce(d3, 3);
+    I->s = 0.8 + 0.6 * u4;
+    I->seed = d5;
+  }
+}
+
+uint64_t or_digest(const int64_t *v, int64_t n) {
+  uint64_t acc = 0;
+  for (int64_t i = 0; i < n; i++) acc += mix64((uint64_t)v[i] ^ (uint64_t)i);
+  return acc;
+}

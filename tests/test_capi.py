@@ -1,0 +1,87 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+header declares with the reference's prototypes, and rejects bad arguments on
+the host before touching a device."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+import bale_old_amd as B
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "bale_hip.h"
+
+# the reference's hot-path prototypes (apps/histo_src/histo.h:47-50,
+# histo_alternates.h:43-46, apps/ig_src/ig.h:49-52, ig_alternates.h:42-45)
+REFERENCE_SYMBOLS = [
+    "histo_agi", "histo_exstack", "histo_exstack2", "histo_conveyor",
+    "histo_exstack2_goto", "histo_exstack2_cyclic", "histo_exstack_function", "histo_exstack2_function",
+    "ig_agi", "ig_exstack", "ig_exstack2", "ig_conveyor",
+    "ig_exstack2_cyclic", "ig_exstack2_goto", "ig_exstack_function", "ig_exstack_pkg",
+]
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", text)) - {"defined", "sizeof"})
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = B.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", str(B.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    decl = declared_functions()
+    assert set(REFERENCE_SYMBOLS) <= set(decl)
+    missing = [f for f in decl if f not in exported]
+    assert not missing, missing
+    for f in decl:
+        getattr(L, f)
+
+
+def test_reference_prototypes_are_kept():
+    text = HEADER.read_text()
+    assert re.search(r"double histo_agi\(int64_t \*index, int64_t len, int64_t \*counts\);", text)
+    assert re.search(r"double ig_agi\(int64_t \*tgt, int64_t \*index, int64_t l_num_req, int64_t \*table\);", text)
+    assert re.search(r"double histo_exstack\(int64_t \*pckindx, int64_t len, int64_t \*lcounts, int64_t buf_cnt\);", text)
+    assert re.search(r"double ig_conveyor\(int64_t \*tgt, int64_t \*pckindx, int64_t l_num_req, int64_t \*ltable\);", text)
+
+
+def test_header_compiles_as_c():
+    src = '#include "bale_hip.h"\nint main(void){return (int)sizeof(bale_config);}\n'
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", str(ROOT / "include"), "-x", "c",
+                        "-", "-fsyntax-only"], input=src, text=True, capture_output=True)
+    assert r.returncode == 0, r.stderr
+    assert C.sizeof(B.BaleConfig) == 24
+
+
+def test_host_side_argument_checks():
+    L = B.lib()
+    assert L.bale_histo(None, -1, None, 10, B.I64, B.PLAIN, None, None) == B.ERR_ARG
+    assert "negative" in B.last_error()
+    assert L.bale_histo(None, 0, None, 10, B.I64, B.PLAIN, None, None) == B.OK  # empty: nothing to do
+    assert L.bale_histo(None, 5, None, 10, B.I64, B.PLAIN, None, None) == B.ERR_ARG
+    assert L.bale_histo(1, 5, 1, 0, B.I64, B.PLAIN, None, None) == B.ERR_ARG
+    assert L.bale_histo(1, 5, 1, 10, 7, B.PLAIN, None, None) == B.ERR_ARG
+    assert L.bale_histo(1, 5, 1, 10, B.I64, 9, None, None) == B.ERR_ARG
+    assert L.bale_gather(None, None, -3, None, 10, B.PLAIN, None, None) == B.ERR_ARG
+    assert L.bale_gather(1, 1, 5, 1, 0, B.PLAIN, None, None) == B.ERR_ARG
+    assert L.bale_gen_index(1, 5, 10, 3, 1.0, 0, 1, None) == B.ERR_ARG
+    assert L.bale_gen_index(1, 5, 10, 1, -1.0, 0, 1, None) == B.ERR_ARG
+    assert L.bale_probe_copy(1, 1, 15, None) == B.ERR_ARG
+
+
+def test_dropin_rejects_negative_length():
+    L = B.lib()
+    assert L.histo_agi(None, -1, None) == -1.0
+    assert L.ig_agi(None, None, -1, None) == -1.0
+
+
+def test_no_device_is_reported():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    assert B.lib().bale_hip_init(0) == B.ERR_NODEV

@@ -1,0 +1,264 @@
+"""GPU parity: libbale_hip.so against the CPU oracle on the same seeded inputs.
+
+Integer results must be bit-exact.  Inputs are generated on the GPU by the
+library's generator, which is itself checked element for element against the
+oracle's generator first.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import bale_old_amd as B
+
+pytestmark = pytest.mark.gpu
+
+
+def to_np(t):
+    return t.cpu().numpy()
+
+
+def gen(dev, n, nbins, dist=0, s=1.0, run_mean=0, seed=1):
+    out = torch.empty(n, dtype=torch.int64, device=dev)
+    B.gen_index(out, nbins, dist=dist, s=s, run_mean=run_mean, seed=seed)
+    return out
+
+
+# ---- generators ------------------------------------------------------------------
+@pytest.mark.parametrize("nbins,dist,s,run_mean,seed", [
+    (1 << 20, 0, 1.0, 0, 1),
+    (1000003, 0, 1.0, 0, 2),
+    (1 << 26, 1, 1.1, 0, 3),
+    (1 << 26, 1, 1.1, 64, 3),
+    (5000, 1, 0.8, 0, 4),
+    (77777, 1, 1.4, 0, 5),
+    (1 << 20, 0, 1.0, 64, 6),
+    (123457, 1, 1.0, 16, 7),
+])
+def test_generator_matches_oracle(dev, orc, nbins, dist, s, run_mean, seed):
+    n = 1 << 18
+    got = to_np(gen(dev, n, nbins, dist, s, run_mean, seed))
+    want = orc.gen_index(n, nbins, dist, s, run_mean, seed)
+    assert np.array_equal(got, want)
+
+
+def test_fill_and_pack_match_oracle(dev, orc):
+    t = torch.empty(100_003, dtype=torch.int64, device=dev)
+    B.fill_splitmix(t)
+    assert np.array_equal(to_np(t), orc.fill_splitmix(100_003))
+    B.fill_neg(t)
+    assert np.array_equal(to_np(t), orc.fill_neg(100_003))
+    idx = gen(dev, 5000, 1 << 30)
+    pck = torch.empty_like(idx)
+    B.pack(pck, idx)
+    assert np.array_equal(to_np(pck), orc.pack(to_np(idx), 1))
+
+
+# ---- histogram -------------------------------------------------------------------
+HISTO_CASES = [
+    # (nbins, n, dist, s, run_mean)
+    (1, 1000, 0, 1.0, 0),
+    (7, 12345, 0, 1.0, 0),
+    (1000, 1 << 18, 1, 1.1, 0),
+    (1 << 15, 1 << 19, 0, 1.0, 0),
+    ((1 << 15) + 3, 1 << 19, 1, 1.3, 0),
+    (1 << 20, 1 << 20, 0, 1.0, 64),
+    (3_000_017, 1 << 20, 1, 1.1, 0),
+]
+STRATS = [(B.DIRECT, 0), (B.DIRECT, 1), (B.DIRECT, 2), (B.LDS, 0), (B.LDS, 1), (B.LDS, 2),
+          (B.BUCKET, 0), (B.AUTO, 0)]
+
+
+@pytest.mark.parametrize("case", HISTO_CASES, ids=lambda c: f"b{c[0]}_n{c[1]}_d{c[2]}_r{c[4]}")
+@pytest.mark.parametrize("dtype", [torch.int64, torch.int32])
+def test_histo_strategies_match_oracle(dev, orc, case, dtype):
+    nbins, n, dist, s, run_mean = case
+    idx = gen(dev, n, nbins, dist, s, run_mean, seed=nbins + n)
+    idx_h = to_np(idx)
+    base = torch.randint(0, 1000, (nbins,), dtype=dtype, device=dev)
+    want = to_np(base).astype(np.int64 if dtype == torch.int64 else np.int32)
+    orc.histo_agi(idx_h, want)
+    for strat, merge in STRATS:
+        for unroll, block in ((4, 256), (16, 1024)):
+            if strat == B.BUCKET and unroll != 4:
+                continue
+            counts = base.clone()
+            cfg = B.config(strat, block=block, unroll=unroll, merge=merge)
+            assert B.histo(idx, counts, cfg=cfg) == 0
+            assert np.array_equal(to_np(counts), want), (strat, merge, unroll, block)
+
+
+@pytest.mark.parametrize("tile_log2", [9, 12, 15])
+def test_histo_tiles(dev, orc, tile_log2):
+    nbins, n = 100_003, 1 << 19
+    idx = gen(dev, n, nbins, 1, 1.05, 8, seed=tile_log2)
+    want = np.bincount(to_np(idx), minlength=nbins)
+    for strat in (B.LDS, B.BUCKET):
+        counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
+        B.histo(idx, counts, cfg=B.config(strat, block=1024, tile_log2=tile_log2))
+        assert np.array_equal(to_np(counts), want), strat
+
+
+def test_histo_packed_and_errors(dev, orc):
+    nbins, n = 4096, 100_000
+    idx = gen(dev, n, nbins, seed=9)
+    pck = torch.empty_like(idx)
+    B.pack(pck, idx)
+    want = np.bincount(to_np(idx), minlength=nbins)
+    for strat in (B.DIRECT, B.LDS, B.BUCKET):
+        counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
+        B.histo(pck, counts, packed=True, cfg=B.config(strat, block=1024))
+        assert np.array_equal(to_np(counts), want)
+    # out-of-range indices are skipped and reported; the rest still count
+    bad = idx.clone()
+    bad[::1000] = nbins + 5
+    bad[1::1000] = -3
+    nbad = 2 * len(range(0, n, 1000))
+    good = to_np(bad)
+    good = good[(good >= 0) & (good < nbins)]
+    for strat in (B.DIRECT, B.LDS, B.BUCKET):
+        counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
+        with pytest.raises(B.BaleError) as ei:
+            B.histo(bad, counts, cfg=B.config(strat, block=1024))
+        assert ei.value.code == B.ERR_RANGE
+        assert str(nbad) in str(ei.value)
+        assert np.array_equal(to_np(counts), np.bincount(good, minlength=nbins))
+    # a packed index naming PE 3 cannot run on one GPU
+    pck2 = pck.clone()
+    pck2[7] |= 3
+    counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
+    with pytest.raises(B.BaleError) as ei:
+        B.histo(pck2, counts, packed=True, cfg=B.config(B.DIRECT))
+    assert ei.value.code == B.ERR_PE
+
+
+def test_histo_empty(dev):
+    counts = torch.zeros(10, dtype=torch.int64, device=dev)
+    idx = torch.empty(0, dtype=torch.int64, device=dev)
+    for strat in (B.DIRECT, B.LDS, B.BUCKET):
+        assert B.histo(idx, counts, cfg=B.config(strat)) == 0
+    assert int(counts.sum()) == 0
+
+
+# ---- gather ----------------------------------------------------------------------
+@pytest.mark.parametrize("tlen,n,dist,s,run_mean", [
+    (1, 100, 0, 1.0, 0),
+    (1000, 12345, 0, 1.0, 0),
+    (1 << 20, 1 << 20, 0, 1.0, 0),
+    (3_000_017, 1 << 20, 1, 1.1, 0),
+    (1 << 22, (1 << 20) + 7, 0, 1.0, 64),
+])
+def test_gather_strategies_match_oracle(dev, orc, tlen, n, dist, s, run_mean):
+    table = torch.empty(tlen, dtype=torch.int64, device=dev)
+    B.fill_splitmix(table)
+    idx = gen(dev, n, tlen, dist, s, run_mean, seed=tlen ^ n)
+    want = orc.ig_agi(to_np(idx), orc.fill_splitmix(tlen))
+    pck = torch.empty_like(idx)
+    B.pack(pck, idx)
+    for strat, blk, unroll, tile in [(B.DIRECT, 256, 16, 0), (B.DIRECT, 1024, 4, 0), (B.DIRECT, 512, 8, 0),
+                                     (B.BUCKET, 1024, 0, 0), (B.BUCKET, 1024, 0, 10), (B.AUTO, 0, 0, 0)]:
+        for packed in (False, True):
+            tgt = torch.full((n,), 7, dtype=torch.int64, device=dev)
+            B.gather(tgt, pck if packed else idx, table, packed=packed,
+                     cfg=B.config(strat, block=blk, unroll=unroll, tile_log2=tile))
+            assert np.array_equal(to_np(tgt), want), (strat, blk, unroll, tile, packed)
+
+
+def test_gather_closed_form_reference_table(dev, orc):
+    tlen, n = 100_000, 1_000_000
+    table = torch.empty(tlen, dtype=torch.int64, device=dev)
+    B.fill_neg(table)  # ig.upc:159-161
+    idx = torch.from_numpy(orc.reference_rand(n, tlen, 5)).to(dev)  # ig.upc:168-170
+    tgt = torch.zeros(n, dtype=torch.int64, device=dev)
+    B.gather(tgt, idx, table)
+    assert torch.equal(tgt, -(idx + 1))  # ig_check_and_zero, ig.upc:96-114
+
+
+def test_gather_range_error(dev):
+    table = torch.arange(100, dtype=torch.int64, device=dev)
+    idx = torch.tensor([0, 5, 100, 99, -1], dtype=torch.int64, device=dev)
+    tgt = torch.zeros(5, dtype=torch.int64, device=dev)
+    with pytest.raises(B.BaleError) as ei:
+        B.gather(tgt, idx, table)
+    assert ei.value.code == B.ERR_RANGE
+    t = to_np(tgt)
+    assert t[0] == 0 and t[1] == 5 and t[3] == 99
+
+
+# ---- reference drop-ins ------------------------------------------------------------
+def ptr(a):
+    return a.ctypes.data if isinstance(a, np.ndarray) else a.data_ptr()
+
+
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_histo_dropins_reference_check(dev, orc, where):
+    """The reference driver's protocol (histo.upc:161-227): every model adds into
+    the same counts; subtracting num_models per index leaves zeros."""
+    L = B.lib()
+    nbins, n = 1000, 1_000_000  # the reference defaults -T 1000 -n 1000000 at one PE
+    idx_h = orc.reference_rand(n, nbins, 120348)
+    pck_h = orc.pack(idx_h, 1)
+    if where == "host":
+        idx, pck, counts = idx_h, pck_h, np.zeros(nbins, np.int64)
+    else:
+        idx, pck = torch.from_numpy(idx_h).to(dev), torch.from_numpy(pck_h).to(dev)
+        counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
+    models = 0
+    assert L.histo_agi(ptr(idx), n, ptr(counts)) >= 0, B.last_error()
+    models += 1
+    for name in B.DROPIN_HISTO_BUF:
+        t = getattr(L, name)(ptr(pck), n, ptr(counts), 1024)
+        assert t >= 0, (name, B.last_error())
+        models += 1
+    assert L.histo_conveyor(ptr(pck), n, ptr(counts)) >= 0, B.last_error()
+    models += 1
+    c = counts if where == "host" else to_np(counts)
+    want = np.bincount(idx_h, minlength=nbins) * models
+    assert np.array_equal(c, want)
+    np.subtract.at(c, idx_h, models)
+    assert not c.any()
+
+
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_ig_dropins_closed_form(dev, orc, where):
+    L = B.lib()
+    tlen, n = 100_000, 1_000_000  # the reference defaults -T 100000 -n 1000000
+    table_h = orc.fill_neg(tlen)
+    idx_h = orc.reference_rand(n, tlen, 5)
+    pck_h = orc.pack(idx_h, 1)
+    if where == "host":
+        table, idx, pck = table_h, idx_h, pck_h
+        mk = lambda: np.zeros(n, np.int64)  # noqa: E731
+        get = lambda t: t  # noqa: E731
+    else:
+        table, idx, pck = (torch.from_numpy(a).to(dev) for a in (table_h, idx_h, pck_h))
+        mk = lambda: torch.zeros(n, dtype=torch.int64, device=dev)  # noqa: E731
+        get = to_np
+    want = -(idx_h + 1)
+    tgt = mk()
+    assert L.ig_agi(ptr(tgt), ptr(idx), n, ptr(table)) >= 0, B.last_error()
+    assert np.array_equal(get(tgt), want)
+    for name in B.DROPIN_IG_BUF + ["ig_conveyor"]:
+        tgt = mk()
+        fn = getattr(L, name)
+        t = fn(ptr(tgt), ptr(pck), n, ptr(table)) if name == "ig_conveyor" else \
+            fn(ptr(tgt), ptr(pck), n, ptr(table), 1024)
+        assert t >= 0, (name, B.last_error())
+        assert np.array_equal(get(tgt), want), name
+
+
+def test_dropin_timing_and_errors(dev, orc):
+    L = B.lib()
+    idx = np.array([0, 1, 2, -4], np.int64)
+    counts = np.zeros(8, np.int64)
+    assert L.histo_agi(ptr(idx), 4, ptr(counts)) == -1.0
+    assert "negative" in B.last_error()
+    pck = np.array([(1 << 16) | 1], np.int64)
+    assert L.histo_exstack(ptr(pck), 1, ptr(counts), 1024) == -1.0
+    assert "PE" in B.last_error()
+    idx = orc.gen_uniform(1 << 20, 1 << 16, 1)
+    counts = np.zeros(1 << 16, np.int64)
+    t = L.histo_agi(ptr(idx), idx.size, ptr(counts))
+    g, h2d, d2h = B.last_timing()
+    assert t == g and t > 0 and h2d > 0 and d2h > 0
