@@ -33,6 +33,8 @@ class BaleConfig(C.Structure):
         ("tile_log2", C.c_int32),
         ("merge", C.c_int32),
         ("groups_per_cu", C.c_int32),
+        ("hot", C.c_int32),
+        ("task_log2", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -42,8 +44,23 @@ class BaleConfig(C.Structure):
         return f"BaleConfig({self.as_dict()})"
 
 
-def config(strategy=AUTO, block=0, unroll=0, tile_log2=0, merge=-1, groups_per_cu=0) -> BaleConfig:
-    return BaleConfig(strategy, block, unroll, tile_log2, merge, groups_per_cu)
+def config(strategy=AUTO, block=0, unroll=0, tile_log2=0, merge=-1, groups_per_cu=0, hot=0,
+           task_log2=0) -> BaleConfig:
+    return BaleConfig(strategy, block, unroll, tile_log2, merge, groups_per_cu, hot, task_log2)
+
+
+class Instance(C.Structure):
+    """bale_instance: one histogram (kind 0) or gather (kind 1) of a batch."""
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("dtype", C.c_int32),
+        ("n", C.c_int64),
+        ("table_len", C.c_int64),
+        ("index", C.c_void_p),
+        ("table", C.c_void_p),
+        ("tgt", C.c_void_p),
+        ("cfg", BaleConfig),
+    ]
 
 
 class BaleError(RuntimeError):
@@ -85,6 +102,19 @@ def _proto(L) -> None:
     f("bale_pack", C.c_int, VP, VP, I64T, VP)
     f("bale_probe_copy", C.c_int, VP, VP, I64T, VP)
     f("bale_probe_random_read", C.c_int, VP, I64T, I64T, VP, VP)
+    IP = C.POINTER(Instance)
+    f("bale_batch_create", VP, IP, I64T, VP)
+    f("bale_batch_launch", C.c_int, VP, VP)
+    f("bale_batch_launch_direct", C.c_int, VP, VP)
+    f("bale_batch_stats", C.c_int, VP, P64, C.c_int)
+    f("bale_batch_destroy", None, VP)
+    f("bale_batch_launch_plain", C.c_int, IP, I64T, VP)
+    f("bale_tune", C.c_int, IP, I64T, I64T, C.POINTER(C.c_float), C.c_int, VP)
+    f("bale_tune_candidates", C.c_int, C.c_int, I64T, C.POINTER(BaleConfig), C.c_int)
+    f("bale_all_alloc", VP, I64T, I64T)
+    f("bale_all_free", None, VP)
+    f("bale_memput", C.c_int, VP, VP, I64T)
+    f("bale_memget", C.c_int, VP, VP, I64T)
     f("histo_agi", C.c_double, VP, I64T, VP)
     f("histo_conveyor", C.c_double, VP, I64T, VP)
     for name in DROPIN_HISTO_BUF:

@@ -88,6 +88,8 @@ struct Device {
   size_t ws_bytes = 0;
 };
 Device* device_state();  // current device's state (created on first use)
+// Record the calling thread's error text (bale_hip_last_error) and return code.
+int bale_set_error(int code, const char* fmt, ...);
 // Workspace of at least `bytes`, 256-byte aligned; nullptr on failure.
 void* workspace(Device* d, size_t bytes, hipStream_t s);
 
@@ -108,7 +110,7 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
                          hipStream_t s);
 
 int launch_histo_direct(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
-                        bool packed, const bale_config& c, ErrWords* err, hipStream_t s);
+                        bool packed, const bale_config& c, Device* d, hipStream_t s);
 int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
                      bool packed, const bale_config& c, Device* d, hipStream_t s);
 int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,

@@ -25,6 +25,16 @@ static int fail(int code, const char* fmt, ...) {
   t_err = buf;
   return code;
 }
+int bale_set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  return code;
+}
+
 static int fail_hip(int rc, const char* what) {
   return fail(BALE_ERR_HIP, "%s: %s", what, hipGetErrorString((hipError_t)rc));
 }
@@ -208,7 +218,7 @@ int bale_histo(const int64_t* index, int64_t n, void* counts, int64_t nbins, int
     int rc;
     switch (c.strategy) {
       case BALE_STRAT_DIRECT:
-        rc = launch_histo_direct(index + off, m, counts, nbins, dtype, packed, c, d->err, s);
+        rc = launch_histo_direct(index + off, m, counts, nbins, dtype, packed, c, d, s);
         break;
       case BALE_STRAT_LDS:
         rc = launch_histo_lds(index + off, m, counts, nbins, dtype, packed, c, d, s);
@@ -278,6 +288,46 @@ int bale_sync_errors(void* stream, int64_t* bad) {
   if (h.range) return fail(BALE_ERR_RANGE, "%llu indices outside the table were skipped",
                            (unsigned long long)h.range);
   return BALE_OK;
+}
+
+void* bale_all_alloc(int64_t nblocks, int64_t blocksize) {
+  if (nblocks < 0 || blocksize <= 0) {
+    fail(BALE_ERR_ARG, "bad allocation size");
+    return nullptr;
+  }
+  if (!device_state()) {
+    fail(BALE_ERR_NODEV, "no usable HIP device");
+    return nullptr;
+  }
+  size_t bytes = (size_t)(nblocks ? nblocks : 1) * (size_t)blocksize;
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    fail(BALE_ERR_ALLOC, "hipMalloc of %zu bytes failed", bytes);
+    return nullptr;
+  }
+  if (hipMemset(p, 0, bytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    (void)hipFree(p);
+    fail(BALE_ERR_HIP, "zeroing the allocation failed");
+    return nullptr;
+  }
+  return p;
+}
+
+void bale_all_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+int bale_memput(void* dst_dev, const void* src_host, int64_t bytes) {
+  if (bytes < 0 || (bytes && (!dst_dev || !src_host))) return fail(BALE_ERR_ARG, "bad copy");
+  int rc = (int)hipMemcpy(dst_dev, src_host, (size_t)bytes, hipMemcpyHostToDevice);
+  return rc ? fail_hip(rc, "memput") : BALE_OK;
+}
+
+int bale_memget(void* dst_host, const void* src_dev, int64_t bytes) {
+  if (bytes < 0 || (bytes && (!dst_host || !src_dev))) return fail(BALE_ERR_ARG, "bad copy");
+  int rc = (int)hipMemcpy(dst_host, src_dev, (size_t)bytes, hipMemcpyDeviceToHost);
+  return rc ? fail_hip(rc, "memget") : BALE_OK;
 }
 
 int bale_fill_splitmix(int64_t* table, int64_t n, void* stream) {

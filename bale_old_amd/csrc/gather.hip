@@ -75,11 +75,13 @@ template <bool PACKED>
 __global__ __launch_bounds__(1024) void k_gather_scatter(const int64_t* __restrict__ idx, int64_t n,
                                                          uint64_t tlen, int shift, int B, int S,
                                                          int64_t chunk,
-                                                         uint32_t* __restrict__ cursor,
+                                                         const uint32_t* __restrict__ rows,
+                                                         const uint32_t* __restrict__ start,
                                                          uint64_t* __restrict__ staged) {
-  part_scatter_body<PACKED, uint64_t>(
-      idx, n, tlen, shift, B, S, chunk, cursor, staged,
-      [](int64_t e, uint64_t, uint64_t low) { return ((uint64_t)e << 32) | low; });
+  part_scatter_body<PACKED, false, false, uint64_t>(
+      blockIdx.x, idx, n, tlen, shift, B, S, chunk, rows, start, nullptr, staged,
+      [](int64_t e, uint64_t, uint64_t low) { return ((uint64_t)e << 32) | low; },
+      [](uint32_t, uint32_t) {});
 }
 
 // Persistent: a workgroup takes a segment of one region's requests and answers
@@ -176,49 +178,46 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
   int shift = c.tile_log2 > 0 ? c.tile_log2 : 18;
   if (shift > 30) shift = 30;
   int64_t B = ceil_div(table_len, (int64_t)1 << shift);
-  while (B > kMaxRegions) {
+  const size_t budget = (size_t)d->lds_per_block;
+  // widen regions until there are at most kMaxRegions and each has an LDS send
+  // buffer of at least 8 entries
+  while (shift < 30 &&
+         (B > kMaxRegions || part_pick_depth((int)B, sizeof(uint64_t), budget, false) < 8)) {
     shift++;
     B = ceil_div(table_len, (int64_t)1 << shift);
   }
-  size_t bytes = part_bytes(B, n, sizeof(uint64_t));
-  void* ws = workspace(d, bytes, s);
+  int S = part_pick_depth((int)B, sizeof(uint64_t), budget, false);
+  if (S == 0) return (int)hipErrorInvalidValue;
+  int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
+  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
+  int64_t chunk = ceil_div(n, C);
+  C = ceil_div(n, chunk);
+  void* ws = workspace(d, part_bytes(C, B, n, sizeof(uint64_t)), s);
   if (!ws) return (int)hipErrorOutOfMemory;
-  PartLayout L = part_carve(ws, B);
-  int rc = launch_zero_bytes(ws, part_zero_bytes(B), s);
-  if (rc) return rc;
-  {
-    int64_t groups = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * 4, ceil_div(n, 4096)));
-    int64_t chunk = ceil_div(n, groups);
-    groups = ceil_div(n, chunk);
-    size_t lds = (size_t)B * 4;
-    if (packed)
-      hipLaunchKernelGGL(k_part_count<true>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
-                         (uint64_t)table_len, shift, (int)B, chunk, L.count, d->err);
-    else
-      hipLaunchKernelGGL(k_part_count<false>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
-                         (uint64_t)table_len, shift, (int)B, chunk, L.count, d->err);
-  }
+  PartLayout L = part_carve(ws, C, B);
+  size_t lds_count = (size_t)B * 4;
+  if (packed)
+    hipLaunchKernelGGL((k_part_count<true, false, false>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
+                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err);
+  else
+    hipLaunchKernelGGL((k_part_count<false, false, false>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
+                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err);
+  hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, s, L.chunk,
+                     (int)C, (int)B, L.total);
   int64_t seg = std::max<int64_t>(4096, ceil_div(n, (int64_t)d->cus * 4));
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.count, (int)B, (uint32_t)seg,
-                     L.start, L.cursor, L.segstart, L.misc);
-  {
-    int S = part_pick_depth((int)B, sizeof(uint64_t), (size_t)d->lds_per_block);
-    if (S == 0) return (int)hipErrorInvalidValue;
-    size_t lds = part_scatter_lds((int)B, S, sizeof(uint64_t));
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
+                     L.start, L.segstart, L.misc);
+  size_t lds = part_scatter_lds((int)B, S, sizeof(uint64_t), false);
+  if (packed) {
     allow_lds(k_gather_scatter<true>, lds);
+    hipLaunchKernelGGL(k_gather_scatter<true>, dim3((unsigned)C), dim3(1024), lds, s, idx, n,
+                       (uint64_t)table_len, shift, (int)B, S, chunk, L.chunk, L.start,
+                       (uint64_t*)L.staged);
+  } else {
     allow_lds(k_gather_scatter<false>, lds);
-    int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
-    int64_t groups = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 1024)));
-    int64_t chunk = ceil_div(n, groups);
-    groups = ceil_div(n, chunk);
-    if (packed)
-      hipLaunchKernelGGL(k_gather_scatter<true>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
-                         (uint64_t)table_len, shift, (int)B, S, chunk, L.cursor,
-                         (uint64_t*)L.staged);
-    else
-      hipLaunchKernelGGL(k_gather_scatter<false>, dim3((unsigned)groups), dim3(1024), lds, s, idx,
-                         n, (uint64_t)table_len, shift, (int)B, S, chunk, L.cursor,
-                         (uint64_t*)L.staged);
+    hipLaunchKernelGGL(k_gather_scatter<false>, dim3((unsigned)C), dim3(1024), lds, s, idx, n,
+                       (uint64_t)table_len, shift, (int)B, S, chunk, L.chunk, L.start,
+                       (uint64_t*)L.staged);
   }
   hipLaunchKernelGGL(k_gather_apply, dim3((unsigned)(d->cus * 2)), dim3(1024), 0, s,
                      (const uint64_t*)L.staged, L.start, L.segstart, L.misc, (int)B, (uint32_t)seg,

@@ -3,6 +3,8 @@
 // Three strategies (the reference's models, re-thought for one chip):
 //   DIRECT  one device-scope atomic per update, the AGI model
 //           (apps/histo_src/histo_agi.upc:61-66, libgetput/libgetput.upc:48-60).
+//           With `hot`, bins a sample sees twice are counted in an LDS hash
+//           table per workgroup first.
 //   LDS     per-workgroup sub-histograms of a bin tile in LDS, flushed with one
 //           device atomic per nonzero bin (updates aggregated before they leave
 //           the CU, as exstack2 aggregates before sending).
@@ -20,7 +22,8 @@ namespace bale {
 
 // ---- wave-level merging of equal bins -------------------------------------
 // key = decoded bin, or ~0 for a lane with nothing to add.  Returns the count
-// this lane must add (0 when another lane adds for it).
+// this lane must add (0 when another lane adds for it).  Every lane of the
+// wave must call it together.
 template <int MERGE>
 __device__ inline uint32_t wave_merge(uint64_t key, bool ok) {
   if (MERGE == 0) return ok ? 1u : 0u;
@@ -76,6 +79,47 @@ __global__ __launch_bounds__(1024) void k_histo_direct(const int64_t* __restrict
   flush_errors(err, bad_r, bad_p);
 }
 
+// DIRECT with hot bins: a bounded grid walks the input; hot bins are counted in
+// the workgroup's LDS hash table and flushed once at the end.
+template <int U, typename CT, bool PACKED>
+__global__ __launch_bounds__(1024) void k_histo_direct_hot(const int64_t* __restrict__ idx, int64_t n,
+                                                           CT* __restrict__ counts, uint64_t nbins,
+                                                           const uint32_t* __restrict__ hot_g,
+                                                           ErrWords* err) {
+  __shared__ uint32_t hkeys[kHotSlots];
+  __shared__ uint32_t hcnt[kHotSlots];
+  const int B = blockDim.x;
+  for (int k = threadIdx.x; k < kHotSlots; k += B) {
+    hkeys[k] = hot_g[k];
+    hcnt[k] = 0;
+  }
+  __syncthreads();
+  unsigned bad_r = 0, bad_p = 0;
+  const int64_t step = (int64_t)gridDim.x * B * U;
+  for (int64_t base = (int64_t)blockIdx.x * B * U + threadIdx.x; base < n; base += step) {
+    int64_t raw[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      int64_t e = base + (int64_t)k * B;
+      raw[k] = e < n ? __builtin_nontemporal_load(idx + e) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      int64_t e = base + (int64_t)k * B;
+      uint64_t j;
+      if (e < n && decode_index<PACKED>(raw[k], nbins, j, bad_r, bad_p)) {
+        int hs = hot_find(hkeys, (uint32_t)j);
+        if (hs >= 0) atomicAdd(&hcnt[hs], 1u);
+        else atomic_add_dev(counts + j, (CT)1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kHotSlots; k += B)
+    if (hcnt[k]) atomic_add_dev(counts + hkeys[k], (CT)hcnt[k]);
+  flush_errors(err, bad_r, bad_p);
+}
+
 // ---- LDS ------------------------------------------------------------------
 // grid = (chunks, tiles); tile t covers bins [t<<tile_bits, +width).
 template <int U, typename CT, bool PACKED, int MERGE>
@@ -122,19 +166,50 @@ __global__ __launch_bounds__(1024) void k_histo_lds(const int64_t* __restrict__ 
 }
 
 // ---- BUCKET ---------------------------------------------------------------
-template <bool PACKED>
+template <bool PACKED, bool HOT, bool RUNS, typename CT>
 __global__ __launch_bounds__(1024) void k_histo_scatter(const int64_t* __restrict__ idx, int64_t n,
                                                         uint64_t nbins, int shift, int B, int S,
-                                                        int64_t chunk, uint32_t* __restrict__ cursor,
-                                                        uint16_t* __restrict__ staged) {
-  part_scatter_body<PACKED, uint16_t>(idx, n, nbins, shift, B, S, chunk, cursor, staged,
-                                      [](int64_t, uint64_t, uint64_t low) { return (uint16_t)low; });
+                                                        int64_t chunk,
+                                                        const uint32_t* __restrict__ rows,
+                                                        const uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ hot_g,
+                                                        uint16_t* __restrict__ staged,
+                                                        CT* __restrict__ counts) {
+  part_scatter_body<PACKED, HOT, RUNS, uint16_t>(
+      blockIdx.x, idx, n, nbins, shift, B, S, chunk, rows, start, hot_g, staged,
+      [](int64_t, uint64_t, uint64_t low) { return (uint16_t)low; },
+      [counts](uint32_t bin, uint32_t c) { atomic_add_dev(counts + bin, (CT)c); });
+}
+
+// Add the 8 16-bit bin offsets packed in x to the LDS histogram.  RUNS: equal
+// neighbours (clustered inputs) are combined into one add.
+template <bool RUNS>
+__device__ inline void lds_add8(uint32_t* sh, u32x4 x) {
+  uint32_t v[8] = {x.x & 0xffff, x.x >> 16, x.y & 0xffff, x.y >> 16,
+                   x.z & 0xffff, x.z >> 16, x.w & 0xffff, x.w >> 16};
+  if (!RUNS) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) atomicAdd(&sh[v[i]], 1u);
+    return;
+  }
+  uint32_t cur = v[0], cnt = 1;
+#pragma unroll
+  for (int i = 1; i < 8; i++) {
+    if (v[i] == cur) {
+      cnt++;
+    } else {
+      atomicAdd(&sh[cur], cnt);
+      cur = v[i];
+      cnt = 1;
+    }
+  }
+  atomicAdd(&sh[cur], cnt);
 }
 
 // Persistent: each workgroup takes segments (<= seg_len entries of one bucket)
 // from a queue, histograms them in LDS and adds the tile into counts.  A bucket
 // with one segment is owned by one workgroup, so its flush is a plain add.
-template <typename CT>
+template <typename CT, bool RUNS>
 __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict__ staged,
                                                       const uint32_t* __restrict__ start,
                                                       const uint32_t* __restrict__ segstart,
@@ -178,15 +253,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
     uint32_t nvec = (e1 - a0) / 8u;
     const u32x4* v = (const u32x4*)(staged + a0);
     for (uint32_t q = tid; q < nvec; q += nthr) {
-      u32x4 x = __builtin_nontemporal_load(v + q);
-      atomicAdd(&sh[x.x & 0xffff], 1u);
-      atomicAdd(&sh[x.x >> 16], 1u);
-      atomicAdd(&sh[x.y & 0xffff], 1u);
-      atomicAdd(&sh[x.y >> 16], 1u);
-      atomicAdd(&sh[x.z & 0xffff], 1u);
-      atomicAdd(&sh[x.z >> 16], 1u);
-      atomicAdd(&sh[x.w & 0xffff], 1u);
-      atomicAdd(&sh[x.w >> 16], 1u);
+      lds_add8<RUNS>(sh, __builtin_nontemporal_load(v + q));
     }
     for (uint32_t k = a0 + nvec * 8u + tid; k < e1; k += nthr) atomicAdd(&sh[staged[k]], 1u);
     __syncthreads();
@@ -207,6 +274,17 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
 }
 
 // ---- launchers ------------------------------------------------------------
+static int norm_block(int b) { return (b == 256 || b == 512 || b == 1024) ? b : 256; }
+static int norm_unroll(int u) { return (u == 4 || u == 8 || u == 16) ? u : 8; }
+
+// Largest bin tile (log2) whose 32-bit LDS counters leave 1 KiB of the
+// per-workgroup LDS for static variables.
+static int max_tile_bits(const Device* d) {
+  int tb = 15;
+  while (tb > 8 && ((size_t)4 << tb) + 1024 > (size_t)d->lds_per_block) tb--;
+  return tb;
+}
+
 template <int U, typename CT, bool PACKED>
 static void go_direct(int merge, dim3 g, dim3 b, hipStream_t s, const int64_t* idx, int64_t n,
                       CT* counts, uint64_t nbins, ErrWords* err) {
@@ -219,33 +297,44 @@ static void go_direct(int merge, dim3 g, dim3 b, hipStream_t s, const int64_t* i
 }
 
 template <typename CT, bool PACKED>
-static void go_direct_u(int U, int merge, dim3 g, dim3 b, hipStream_t s, const int64_t* idx,
-                        int64_t n, CT* counts, uint64_t nbins, ErrWords* err) {
-  if (U == 16) go_direct<16, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, err);
-  else if (U == 8) go_direct<8, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, err);
-  else go_direct<4, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, err);
-}
-
-static int norm_block(int b) { return (b == 256 || b == 512 || b == 1024) ? b : 256; }
-static int norm_unroll(int u) { return (u == 4 || u == 8 || u == 16) ? u : 8; }
-
-int launch_histo_direct(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
-                        bool packed, const bale_config& c, ErrWords* err, hipStream_t s) {
-  if (n <= 0) return 0;
+static int direct_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins,
+                        const bale_config& c, Device* d, hipStream_t s) {
   int block = norm_block(c.block), U = norm_unroll(c.unroll);
   int merge = c.merge < 0 ? 0 : c.merge;
   int64_t per = (int64_t)block * U;
   int64_t grid = ceil_div(n, per);
+  if (c.hot && nbins < (int64_t)kHotEmpty) {
+    void* ws = workspace(d, kHotSlots * 4, s);
+    if (!ws) return (int)hipErrorOutOfMemory;
+    uint32_t* hot = (uint32_t*)ws;
+    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins, hot);
+    int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : (block == 1024 ? 2 : 4);
+    grid = std::min<int64_t>(grid, (int64_t)d->cus * gpc);
+    dim3 g((unsigned)grid), b(block);
+    if (U == 16)
+      hipLaunchKernelGGL((k_histo_direct_hot<16, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, d->err);
+    else if (U == 8)
+      hipLaunchKernelGGL((k_histo_direct_hot<8, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, d->err);
+    else
+      hipLaunchKernelGGL((k_histo_direct_hot<4, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, d->err);
+    return (int)hipGetLastError();
+  }
   if (grid > 0x7fffffff) return (int)hipErrorInvalidValue;
   dim3 g((unsigned)grid), b(block);
-  if (dtype == BALE_I64) {
-    if (packed) go_direct_u<int64_t, true>(U, merge, g, b, s, idx, n, (int64_t*)counts, nbins, err);
-    else go_direct_u<int64_t, false>(U, merge, g, b, s, idx, n, (int64_t*)counts, nbins, err);
-  } else {
-    if (packed) go_direct_u<int32_t, true>(U, merge, g, b, s, idx, n, (int32_t*)counts, nbins, err);
-    else go_direct_u<int32_t, false>(U, merge, g, b, s, idx, n, (int32_t*)counts, nbins, err);
-  }
+  if (U == 16) go_direct<16, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, d->err);
+  else if (U == 8) go_direct<8, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, d->err);
+  else go_direct<4, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, d->err);
   return (int)hipGetLastError();
+}
+
+int launch_histo_direct(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
+                        bool packed, const bale_config& c, Device* d, hipStream_t s) {
+  if (n <= 0) return 0;
+  if (dtype == BALE_I64)
+    return packed ? direct_typed<int64_t, true>(idx, n, (int64_t*)counts, nbins, c, d, s)
+                  : direct_typed<int64_t, false>(idx, n, (int64_t*)counts, nbins, c, d, s);
+  return packed ? direct_typed<int32_t, true>(idx, n, (int32_t*)counts, nbins, c, d, s)
+                : direct_typed<int32_t, false>(idx, n, (int32_t*)counts, nbins, c, d, s);
 }
 
 template <int U, typename CT, bool PACKED>
@@ -270,14 +359,6 @@ static void go_lds_u(int U, int merge, dim3 g, dim3 b, size_t lds, hipStream_t s
   if (U == 16) go_lds<16, CT, PACKED>(merge, g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
   else if (U == 8) go_lds<8, CT, PACKED>(merge, g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
   else go_lds<4, CT, PACKED>(merge, g, b, lds, s, idx, n, counts, nbins, tb, chunk, err);
-}
-
-// Largest bin tile (log2) whose 32-bit LDS counters leave 1 KiB of the
-// per-workgroup LDS for static variables.
-static int max_tile_bits(const Device* d) {
-  int tb = 15;
-  while (tb > 8 && ((size_t)4 << tb) + 1024 > (size_t)d->lds_per_block) tb--;
-  return tb;
 }
 
 int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
@@ -313,73 +394,75 @@ int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins,
 
 constexpr int kMaxBuckets = 4096;
 
-int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
-                        bool packed, const bale_config& c, Device* d, hipStream_t s) {
-  if (n <= 0) return 0;
-  if (n >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;  // caller splits
+template <bool PACKED, bool HOT, bool RUNS, typename CT>
+static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins,
+                        const bale_config& c, Device* d, hipStream_t s) {
   int tb = c.tile_log2 > 0 ? c.tile_log2 : max_tile_bits(d);
   if (tb > max_tile_bits(d)) tb = max_tile_bits(d);
   if (tb > 16) tb = 16;  // entries are 16-bit bin offsets
   int64_t B = ceil_div(nbins, (int64_t)1 << tb);
   if (B > kMaxBuckets) return (int)hipErrorInvalidValue;
-  size_t bytes = part_bytes(B, n, sizeof(uint16_t));
-  void* ws = workspace(d, bytes, s);
-  if (!ws) return (int)hipErrorOutOfMemory;
-  PartLayout L = part_carve(ws, B);
-  int rc = launch_zero_bytes(ws, part_zero_bytes(B), s);
-  if (rc) return rc;
+  const size_t budget = (size_t)d->lds_per_block;
+  int S = part_pick_depth((int)B, sizeof(uint16_t), budget, HOT);
+  if (S == 0) return (int)hipErrorInvalidValue;
   int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
-  // pass 1: count
-  {
-    int64_t groups = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * 4, ceil_div(n, 4096)));
-    int64_t chunk = ceil_div(n, groups);
-    groups = ceil_div(n, chunk);
-    size_t lds = (size_t)B * 4;
-    if (packed)
-      hipLaunchKernelGGL(k_part_count<true>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
-                         (uint64_t)nbins, tb, (int)B, chunk, L.count, d->err);
-    else
-      hipLaunchKernelGGL(k_part_count<false>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
-                         (uint64_t)nbins, tb, (int)B, chunk, L.count, d->err);
-  }
+  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
+  int64_t chunk = ceil_div(n, C);
+  C = ceil_div(n, chunk);
+  void* ws = workspace(d, part_bytes(C, B, n, sizeof(uint16_t)), s);
+  if (!ws) return (int)hipErrorOutOfMemory;
+  PartLayout L = part_carve(ws, C, B);
+  if (HOT)
+    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins, L.hot);
+  size_t lds_count = (size_t)(B + (HOT ? kHotSlots : 0)) * 4;
+  hipLaunchKernelGGL((k_part_count<PACKED, HOT, RUNS>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
+                     n, (uint64_t)nbins, tb, (int)B, chunk, L.chunk, L.hot, d->err);
+  hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, s, L.chunk,
+                     (int)C, (int)B, L.total);
   // segments: enough of them to spread the apply pass over the chip
-  int64_t seg = std::max<int64_t>(4096, ceil_div(n, (int64_t)d->cus * 2));
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.count, (int)B, (uint32_t)seg,
-                     L.start, L.cursor, L.segstart, L.misc);
-  // pass 3: scatter through LDS send buffers
-  {
-    size_t budget = (size_t)d->lds_per_block;
-    int S = part_pick_depth((int)B, sizeof(uint16_t), budget);
-    if (S == 0) return (int)hipErrorInvalidValue;
-    size_t lds = part_scatter_lds((int)B, S, sizeof(uint16_t));
-    allow_lds(k_histo_scatter<true>, lds);
-    allow_lds(k_histo_scatter<false>, lds);
-    int64_t groups = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 1024)));
-    int64_t chunk = ceil_div(n, groups);
-    groups = ceil_div(n, chunk);
-    if (packed)
-      hipLaunchKernelGGL(k_histo_scatter<true>, dim3((unsigned)groups), dim3(1024), lds, s, idx, n,
-                         (uint64_t)nbins, tb, (int)B, S, chunk, L.cursor, (uint16_t*)L.staged);
-    else
-      hipLaunchKernelGGL(k_histo_scatter<false>, dim3((unsigned)groups), dim3(1024), lds, s, idx,
-                         n, (uint64_t)nbins, tb, (int)B, S, chunk, L.cursor, (uint16_t*)L.staged);
-  }
-  // apply
-  {
-    size_t lds = (size_t)4 << tb;
-    int64_t groups = (int64_t)d->cus * (lds <= 65536 ? 2 : 1);
-    allow_lds(k_histo_apply<int64_t>, lds);
-    allow_lds(k_histo_apply<int32_t>, lds);
-    if (dtype == BALE_I64)
-      hipLaunchKernelGGL(k_histo_apply<int64_t>, dim3((unsigned)groups), dim3(1024), lds, s,
-                         (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B,
-                         (uint32_t)seg, tb, (uint64_t)nbins, (int64_t*)counts);
-    else
-      hipLaunchKernelGGL(k_histo_apply<int32_t>, dim3((unsigned)groups), dim3(1024), lds, s,
-                         (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B,
-                         (uint32_t)seg, tb, (uint64_t)nbins, (int32_t*)counts);
-  }
+  // Long enough that a segment's flush (up to one add per bin of the tile) stays
+  // a fraction of its updates, short enough to spread the pass over the chip.
+  int64_t seg = std::max<int64_t>({4096, ceil_div(n, (int64_t)d->cus * 2), (int64_t)4 << tb});
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
+                     L.start, L.segstart, L.misc);
+  size_t lds = part_scatter_lds((int)B, S, sizeof(uint16_t), HOT);
+  allow_lds(k_histo_scatter<PACKED, HOT, RUNS, CT>, lds);
+  hipLaunchKernelGGL((k_histo_scatter<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024), lds, s, idx,
+                     n, (uint64_t)nbins, tb, (int)B, S, chunk, L.chunk, L.start, L.hot,
+                     (uint16_t*)L.staged, counts);
+  size_t lds_apply = (size_t)4 << tb;
+  int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
+  allow_lds(k_histo_apply<CT, RUNS>, lds_apply);
+  hipLaunchKernelGGL((k_histo_apply<CT, RUNS>), dim3((unsigned)groups), dim3(1024), lds_apply, s,
+                     (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B, (uint32_t)seg,
+                     tb, (uint64_t)nbins, counts);
   return (int)hipGetLastError();
+}
+
+template <typename CT>
+static int bucket_dispatch(const int64_t* idx, int64_t n, CT* counts, int64_t nbins, bool packed,
+                           const bale_config& c, Device* d, hipStream_t s) {
+  const bool hot = c.hot != 0, runs = c.merge > 0;
+  const int sel = (packed ? 4 : 0) | (hot ? 2 : 0) | (runs ? 1 : 0);
+  switch (sel) {
+    case 0: return bucket_typed<false, false, false, CT>(idx, n, counts, nbins, c, d, s);
+    case 1: return bucket_typed<false, false, true, CT>(idx, n, counts, nbins, c, d, s);
+    case 2: return bucket_typed<false, true, false, CT>(idx, n, counts, nbins, c, d, s);
+    case 3: return bucket_typed<false, true, true, CT>(idx, n, counts, nbins, c, d, s);
+    case 4: return bucket_typed<true, false, false, CT>(idx, n, counts, nbins, c, d, s);
+    case 5: return bucket_typed<true, false, true, CT>(idx, n, counts, nbins, c, d, s);
+    case 6: return bucket_typed<true, true, false, CT>(idx, n, counts, nbins, c, d, s);
+    default: return bucket_typed<true, true, true, CT>(idx, n, counts, nbins, c, d, s);
+  }
+}
+
+int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
+                        bool packed, const bale_config& c, Device* d, hipStream_t s) {
+  if (n <= 0) return 0;
+  if (n >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;  // caller splits
+  if (dtype == BALE_I64)
+    return bucket_dispatch<int64_t>(idx, n, (int64_t*)counts, nbins, packed, c, d, s);
+  return bucket_dispatch<int32_t>(idx, n, (int32_t*)counts, nbins, packed, c, d, s);
 }
 
 }  // namespace bale

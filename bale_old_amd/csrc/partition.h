@@ -4,169 +4,394 @@
 // This is the on-chip form of bale's aggregate-by-destination idea (exstack /
 // conveyor: push an item into the send buffer of its destination, ship the
 // buffer when it is full).  Here a "destination" is a bucket of 2^shift
-// consecutive bins or table entries.  Each workgroup keeps one small LDS send
-// buffer per bucket; an item is appended to its bucket's buffer, and a full
-// buffer is shipped to the bucket's region of a global staging array with one
-// coalesced write.  Regions are sized exactly by a counting pass, so the
-// shipping cursor (one device atomic per buffer) never overflows.
+// consecutive bins or table entries, and a "PE" is a workgroup working on a
+// fixed chunk of the input.
 //
-//   pass 1  k_part_count   per-bucket totals (LDS histogram, one atomic per bucket)
-//   pass 2  k_part_scan    exclusive scan -> region starts, cursors, segment list
-//   pass 3  k_part_scatter LDS send buffers, coalesced flushes
+//   k_part_count    per-(chunk, bucket) counts, one row per workgroup, no atomics
+//   k_part_offsets  column prefix over chunks: each chunk's first slot per bucket
+//   k_part_scan     prefix over buckets: bucket starts and the apply segment list
+//   part_scatter    one LDS send buffer per bucket; a full buffer is shipped with
+//                   one coalesced write to the chunk's own, exactly sized range of
+//                   the bucket (LDS cursor, no device atomics)
+//
+// Hot bins (histogram only): a sample of the input is sorted in LDS and bins
+// seen at least twice become "hot"; they go to an LDS hash table in every
+// workgroup (counted there, flushed once per workgroup) and skip partitioning.
+// Zipf-like inputs put most of their mass on such bins.
 #pragma once
 #include "bale_internal.h"
 
 namespace bale {
 
-// Workspace layout of one partition (all 32-bit words, then the staging array).
+constexpr int kHotSlots = 2048;          // LDS hash slots (power of two)
+constexpr int kHotMax = 1024;            // at most this many hot bins
+constexpr uint32_t kHotEmpty = 0xFFFFFFFFu;
+constexpr int kSketch = 4096;            // sampled indices for the hot set
+
+__device__ inline uint32_t hot_hash(uint32_t key) { return (key * 0x9E3779B1u) >> (32 - 11); }
+
+// Slot of `key` in the hash table `keys`, or -1.
+__device__ inline int hot_find(const uint32_t* keys, uint32_t key) {
+  uint32_t h = hot_hash(key);
+  for (int probe = 0; probe < kHotSlots; probe++) {
+    uint32_t k = keys[h];
+    if (k == key) return (int)h;
+    if (k == kHotEmpty) return -1;
+    h = (h + 1) & (kHotSlots - 1);
+  }
+  return -1;
+}
+
+// Workspace layout (32-bit words, then the staging array).
 struct PartLayout {
-  uint32_t* count;     // [B]   items per bucket
-  uint32_t* start;     // [B+1] region start per bucket
-  uint32_t* cursor;    // [B]   shipping cursor per bucket
-  uint32_t* segstart;  // [B+1] first segment of each bucket
+  uint32_t* chunk;     // [C*B] per-chunk counts, then per-chunk first slots
+  uint32_t* start;     // [B+1] bucket starts
+  uint32_t* total;     // [B]   items per bucket
+  uint32_t* segstart;  // [B+1] first apply segment of each bucket
   uint32_t* misc;      // [4]   [0] total segments, [1] work-queue head
+  uint32_t* hot;       // [kHotSlots] hot-bin hash table (keys)
   void* staged;        // [n]   entries grouped by bucket
 };
 
-// Carve `ws` into a PartLayout for B buckets and n entries of `esize` bytes.
-inline size_t part_bytes(int64_t B, int64_t n, size_t esize) {
-  size_t words = (size_t)(B + (B + 1) + B + (B + 1) + 4);
-  size_t head = ((words * 4) + 255) & ~(size_t)255;
+inline size_t part_words(int64_t C, int64_t B) {
+  return (size_t)(C * B + (B + 1) + B + (B + 1) + 4 + kHotSlots);
+}
+inline size_t part_bytes(int64_t C, int64_t B, int64_t n, size_t esize) {
+  size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
   return head + (size_t)n * esize + 256;
 }
-inline PartLayout part_carve(void* ws, int64_t B) {
+inline PartLayout part_carve(void* ws, int64_t C, int64_t B) {
   PartLayout L;
   uint32_t* w = (uint32_t*)ws;
-  L.count = w;
-  L.start = L.count + B;
-  L.cursor = L.start + (B + 1);
-  L.segstart = L.cursor + B;
+  L.chunk = w;
+  L.start = L.chunk + C * B;
+  L.total = L.start + (B + 1);
+  L.segstart = L.total + B;
   L.misc = L.segstart + (B + 1);
-  size_t words = (size_t)(B + (B + 1) + B + (B + 1) + 4);
-  size_t head = ((words * 4) + 255) & ~(size_t)255;
+  L.hot = L.misc + 4;
+  size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
   L.staged = (char*)ws + head;
   return L;
 }
-inline size_t part_zero_bytes(int64_t B) { return (size_t)(B + (B + 1) + B + (B + 1) + 4) * 4; }
 
-// ---- pass 1 -----------------------------------------------------------------
+// ---- hot-set sketch (one workgroup of 1024) -----------------------------------
+// Samples kSketch indices at a fixed stride, sorts them in LDS and writes the
+// hash table of bins that occur at least twice (at most kHotMax of them).
 template <bool PACKED>
-__global__ __launch_bounds__(1024) void k_part_count(const int64_t* __restrict__ idx, int64_t n,
-                                                     uint64_t limit, int shift, int B,
-                                                     int64_t chunk, uint32_t* __restrict__ count,
-                                                     ErrWords* err) {
-  extern __shared__ uint32_t hist[];
-  for (int b = threadIdx.x; b < B; b += blockDim.x) hist[b] = 0;
-  __syncthreads();
-  int64_t lo = (int64_t)blockIdx.x * chunk;
-  int64_t hi = lo + chunk < n ? lo + chunk : n;
-  unsigned bad_r = 0, bad_p = 0;
-  for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+__device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t n, uint64_t limit,
+                                       uint32_t* __restrict__ hot_out) {
+  __shared__ uint32_t s[kSketch];
+  __shared__ uint32_t table[kHotSlots];
+  __shared__ uint32_t nhot;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < kSketch; k += 1024) {
+    int64_t pos = (int64_t)(((unsigned __int128)k * (uint64_t)n) / kSketch);
     uint64_t j;
-    if (decode_index<PACKED>(__builtin_nontemporal_load(idx + e), limit, j, bad_r, bad_p))
-      atomicAdd(&hist[j >> shift], 1u);
+    unsigned br = 0, bp = 0;
+    s[k] = (pos < n && decode_index<PACKED>(idx[pos], limit, j, br, bp) && j < kHotEmpty)
+               ? (uint32_t)j : kHotEmpty;
+  }
+  for (int k = tid; k < kHotSlots; k += 1024) table[k] = kHotEmpty;
+  if (tid == 0) nhot = 0;
+  __syncthreads();
+  // bitonic sort of s[0..kSketch)
+  for (int size = 2; size <= kSketch; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = tid; k < kSketch / 2; k += 1024) {
+        int lo = 2 * k - (k & (stride - 1));
+        int hi = lo + stride;
+        bool up = (lo & size) == 0;
+        uint32_t a = s[lo], b = s[hi];
+        if ((a > b) == up) {
+          s[lo] = b;
+          s[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // a bin is hot when it occurs at least twice in the sample
+  for (int k = tid; k < kSketch - 1; k += 1024) {
+    uint32_t v = s[k];
+    bool first = (k == 0 || s[k - 1] != v);
+    if (v != kHotEmpty && first && s[k + 1] == v) {
+      if (atomicAdd(&nhot, 1u) < (uint32_t)kHotMax) {
+        uint32_t h = hot_hash(v);
+        while (atomicCAS(&table[h], kHotEmpty, v) != kHotEmpty) h = (h + 1) & (kHotSlots - 1);
+      }
+    }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < B; b += blockDim.x)
-    if (hist[b]) atomicAdd(&count[b], hist[b]);
+  for (int k = tid; k < kHotSlots; k += 1024) hot_out[k] = table[k];
+  __syncthreads();
+}
+
+template <bool PACKED>
+__global__ __launch_bounds__(1024) void k_hot_sketch(const int64_t* __restrict__ idx, int64_t n,
+                                                     uint64_t limit, uint32_t* __restrict__ hot_out) {
+  hot_sketch_body<PACKED>(idx, n, limit, hot_out);
+}
+
+// Runs of equal keys in adjacent lanes of the wave.  Returns the lane that heads
+// this lane's run and sets len to the run's length.  Every lane of the wave
+// must call it together.
+__device__ inline int wave_runs(uint32_t key, uint32_t& len) {
+  const int lane = __lane_id();
+  uint32_t prev = __shfl_up(key, 1);
+  bool change = (lane == 0) || (prev != key);
+  uint64_t chg = __ballot(change);
+  uint64_t upto = lane == 63 ? ~0ULL : ((2ULL << lane) - 1);  // lanes <= this one
+  int head = 63 - __builtin_clzll(chg & upto);
+  uint64_t later = chg & ~upto;
+  int end = later ? __builtin_ctzll(later) : 64;
+  len = (uint32_t)(end - head);
+  return head;
+}
+
+// ---- pass 1: per-chunk bucket counts ---------------------------------------
+// Counts chunk c's items per bucket into row c of `rows`.  sh: LDS of
+// (B + kHotSlots) words.  Called by a whole workgroup.
+template <bool PACKED, bool HOT, bool RUNS>
+__device__ inline void part_count_body(uint32_t* sh, int64_t c, const int64_t* __restrict__ idx,
+                                       int64_t n, uint64_t limit, int shift, int B, int64_t chunk,
+                                       uint32_t* __restrict__ rows,
+                                       const uint32_t* __restrict__ hot_g, ErrWords* err) {
+  uint32_t* hist = sh;              // [B]
+  uint32_t* hot = sh + B;           // [kHotSlots] when HOT
+  for (int b = threadIdx.x; b < B; b += blockDim.x) hist[b] = 0;
+  if (HOT)
+    for (int k = threadIdx.x; k < kHotSlots; k += blockDim.x) hot[k] = hot_g[k];
+  __syncthreads();
+  int64_t lo = c * chunk;
+  int64_t hi = lo + chunk < n ? lo + chunk : n;
+  unsigned bad_r = 0, bad_p = 0;
+  constexpr int U = 4;
+  const int nthr = blockDim.x;
+  // uniform loop bound: all lanes of a wave reach wave_runs together
+  for (int64_t base = lo; base < hi; base += (int64_t)nthr * U) {
+    int64_t raw[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int64_t e = base + (int64_t)u * nthr + threadIdx.x;
+      raw[u] = e < hi ? __builtin_nontemporal_load(idx + e) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int64_t e = base + (int64_t)u * nthr + threadIdx.x;
+      uint32_t key = kHotEmpty;
+      uint64_t j;
+      if (e < hi && decode_index<PACKED>(raw[u], limit, j, bad_r, bad_p))
+        if (!HOT || hot_find(hot, (uint32_t)j) < 0) key = (uint32_t)(j >> shift);
+      if (RUNS) {
+        uint32_t len;
+        int head = wave_runs(key, len);
+        if (key != kHotEmpty && __lane_id() == head) atomicAdd(&hist[key], len);
+      } else if (key != kHotEmpty) {
+        atomicAdd(&hist[key], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t* row = rows + (size_t)c * B;
+  for (int b = threadIdx.x; b < B; b += blockDim.x) row[b] = hist[b];
   flush_errors(err, bad_r, bad_p);
 }
 
-// ---- pass 2 (one workgroup of 1024) -------------------------------------------
-static __global__ __launch_bounds__(1024) void k_part_scan(const uint32_t* __restrict__ count, int B,
-                                                    uint32_t seg_len, uint32_t* __restrict__ start,
-                                                    uint32_t* __restrict__ cursor,
-                                                    uint32_t* __restrict__ segstart,
-                                                    uint32_t* __restrict__ misc) {
+template <bool PACKED, bool HOT, bool RUNS>
+__global__ __launch_bounds__(1024) void k_part_count(const int64_t* __restrict__ idx, int64_t n,
+                                                     uint64_t limit, int shift, int B,
+                                                     int64_t chunk, uint32_t* __restrict__ rows,
+                                                     const uint32_t* __restrict__ hot_g,
+                                                     ErrWords* err) {
+  extern __shared__ uint32_t sh[];
+  part_count_body<PACKED, HOT, RUNS>(sh, blockIdx.x, idx, n, limit, shift, B, chunk, rows, hot_g,
+                                     err);
+}
+
+// ---- pass 2a: column prefix over chunks (thread per bucket) ----------------------
+static __global__ __launch_bounds__(256) void k_part_offsets(uint32_t* __restrict__ rows, int C,
+                                                             int B, uint32_t* __restrict__ total) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  uint32_t run = 0;
+  for (int c = 0; c < C; c++) {
+    uint32_t v = rows[(size_t)c * B + b];
+    rows[(size_t)c * B + b] = run;
+    run += v;
+  }
+  total[b] = run;
+}
+
+// ---- pass 2b: prefix over buckets (one workgroup, any size up to 1024) ------------
+// start[b] = items before bucket b; segstart[b] = apply segments before bucket
+// b; misc[0] = total segments, misc[1] = 0 (the apply queue head).
+__device__ inline void part_scan_body(const uint32_t* __restrict__ count, int B, uint32_t seg_len,
+                                      uint32_t* __restrict__ start,
+                                      uint32_t* __restrict__ segstart,
+                                      uint32_t* __restrict__ misc) {
   __shared__ uint32_t part_a[1024], part_b[1024];
-  // each thread owns a contiguous run of buckets
-  int per = (B + 1023) / 1024;
-  int b0 = threadIdx.x * per;
+  const int T = blockDim.x, t = threadIdx.x;
+  int per = (B + T - 1) / T;
+  int b0 = t * per;
   uint32_t sa = 0, sb = 0;
   for (int b = b0; b < b0 + per && b < B; b++) {
     sa += count[b];
     sb += (count[b] + seg_len - 1) / seg_len;
   }
-  part_a[threadIdx.x] = sa;
-  part_b[threadIdx.x] = sb;
+  part_a[t] = sa;
+  part_b[t] = sb;
   __syncthreads();
-  // Hillis-Steele inclusive scan over the 1024 partials
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t va = threadIdx.x >= off ? part_a[threadIdx.x - off] : 0;
-    uint32_t vb = threadIdx.x >= off ? part_b[threadIdx.x - off] : 0;
+  // Hillis-Steele inclusive scan over the T partials
+  for (int off = 1; off < T; off <<= 1) {
+    uint32_t va = t >= off ? part_a[t - off] : 0;
+    uint32_t vb = t >= off ? part_b[t - off] : 0;
     __syncthreads();
-    part_a[threadIdx.x] += va;
-    part_b[threadIdx.x] += vb;
+    part_a[t] += va;
+    part_b[t] += vb;
     __syncthreads();
   }
-  uint32_t ra = threadIdx.x ? part_a[threadIdx.x - 1] : 0;
-  uint32_t rb = threadIdx.x ? part_b[threadIdx.x - 1] : 0;
+  uint32_t ra = t ? part_a[t - 1] : 0;
+  uint32_t rb = t ? part_b[t - 1] : 0;
   for (int b = b0; b < b0 + per && b < B; b++) {
     start[b] = ra;
-    cursor[b] = ra;
     segstart[b] = rb;
     ra += count[b];
     rb += (count[b] + seg_len - 1) / seg_len;
   }
-  if (threadIdx.x == 1023) {
-    start[B] = part_a[1023];
-    segstart[B] = part_b[1023];
-    misc[0] = part_b[1023];
+  if (t == T - 1) {
+    start[B] = part_a[T - 1];
+    segstart[B] = part_b[T - 1];
+    misc[0] = part_b[T - 1];
     misc[1] = 0;
   }
+  __syncthreads();
 }
 
-// ---- pass 3 -----------------------------------------------------------------
-// MakeEntry(e, j, lo_bits) -> entry value stored for element e with decoded
-// index j (lo_bits = j & ((1<<shift)-1)).
-template <bool PACKED, typename E, typename MakeEntry>
-__device__ inline void part_scatter_body(const int64_t* __restrict__ idx, int64_t n,
+static __global__ __launch_bounds__(1024) void k_part_scan(const uint32_t* __restrict__ count, int B,
+                                                           uint32_t seg_len,
+                                                           uint32_t* __restrict__ start,
+                                                           uint32_t* __restrict__ segstart,
+                                                           uint32_t* __restrict__ misc) {
+  part_scan_body(count, B, seg_len, start, segstart, misc);
+}
+
+// ---- pass 3: scatter through LDS send buffers -------------------------------------
+// LDS layout: cursor[B], fill[B], list[B], nlist, [hot keys, hot counts], buf[B*S].
+inline size_t part_scatter_lds(int B, int S, size_t esize, bool hot) {
+  size_t head = (size_t)(3 * B + 1) * 4 + (hot ? (size_t)2 * kHotSlots * 4 : 0);
+  return ((head + 15) & ~(size_t)15) + (size_t)B * S * esize;
+}
+// Largest power-of-two buffer depth S in [4, 256] that fits `budget` bytes (0 if none).
+inline int part_pick_depth(int B, size_t esize, size_t budget, bool hot) {
+  int S = 256;
+  while (S > 4 && part_scatter_lds(B, S, esize, hot) > budget) S >>= 1;
+  return part_scatter_lds(B, S, esize, hot) <= budget ? S : 0;
+}
+
+constexpr int kScatterU = 4;  // elements per thread per round
+
+// MakeEntry(e, j, low) -> entry stored for element e with decoded index j
+// (low = j & ((1<<shift)-1)).  HotFlush(bin, count) flushes one hot counter.
+// RUNS: adjacent lanes with the same bucket (or hot bin) reserve their slots
+// with one LDS atomic per run (clustered inputs).
+template <bool PACKED, bool HOT, bool RUNS, typename E, typename MakeEntry, typename HotFlush>
+__device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ idx, int64_t n,
                                          uint64_t limit, int shift, int B, int S, int64_t chunk,
-                                         uint32_t* __restrict__ cursor, E* __restrict__ staged,
-                                         MakeEntry make) {
+                                         const uint32_t* __restrict__ rows,
+                                         const uint32_t* __restrict__ start,
+                                         const uint32_t* __restrict__ hot_g,
+                                         E* __restrict__ staged, MakeEntry make,
+                                         HotFlush hot_flush) {
   extern __shared__ __align__(16) unsigned char smem[];
-  uint32_t* fill = (uint32_t*)smem;               // [B]
-  uint32_t* list = fill + B;                       // [B] buckets to ship this step
-  uint32_t* nlist = list + B;                      // [1]
-  E* buf = (E*)(smem + (((size_t)(2 * B + 1) * 4 + 15) & ~(size_t)15));  // [B*S]
+  uint32_t* cursor = (uint32_t*)smem;  // [B] next staging slot of this chunk per bucket
+  uint32_t* fill = cursor + B;         // [B]
+  uint32_t* list = fill + B;           // [B] buckets to ship this step
+  uint32_t* nlist = list + B;          // [1]
+  uint32_t* hkeys = nlist + 1;         // [kHotSlots]
+  uint32_t* hcnt = hkeys + kHotSlots;  // [kHotSlots]
+  size_t head = (size_t)(3 * B + 1) * 4 + (HOT ? (size_t)2 * kHotSlots * 4 : 0);
+  E* buf = (E*)(smem + ((head + 15) & ~(size_t)15));  // [B*S]
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = __lane_id(), wave = tid >> 6, nwaves = nthr >> 6;
-  for (int b = tid; b < B; b += nthr) fill[b] = 0;
+  const uint32_t* row = rows + (size_t)c * B;
+  for (int b = tid; b < B; b += nthr) {
+    cursor[b] = start[b] + row[b];
+    fill[b] = 0;
+  }
+  if (HOT)
+    for (int k = tid; k < kHotSlots; k += nthr) {
+      hkeys[k] = hot_g[k];
+      hcnt[k] = 0;
+    }
   if (tid == 0) *nlist = 0;
   __syncthreads();
   const uint64_t mask = (1ULL << shift) - 1;
-  int64_t lo = (int64_t)blockIdx.x * chunk;
+  int64_t lo = c * chunk;
   int64_t hi = lo + chunk < n ? lo + chunk : n;
 
-  // ship the first `cnt` entries of bucket b with one wave (lanes stride the entries)
+  // one wave ships the first `cnt` entries of bucket b to the chunk's range
   auto ship = [&](int b, uint32_t cnt) {
     uint32_t pos = 0;
-    if (lane == 0) pos = atomicAdd(&cursor[b], cnt);
+    if (lane == 0) pos = atomicAdd(&cursor[b], cnt);  // LDS atomic
     pos = __shfl(pos, 0);
     for (uint32_t k = lane; k < cnt; k += 64) staged[pos + k] = buf[(size_t)b * S + k];
   };
 
-  for (int64_t base = lo; base < hi; base += nthr) {
-    int64_t e = base + tid;
-    bool pending = false;
-    int b = 0;
-    E ent{};
-    if (e < hi) {
-      uint64_t j;
-      unsigned br = 0, bp = 0;
-      if (decode_index<PACKED>(__builtin_nontemporal_load(idx + e), limit, j, br, bp)) {
-        b = (int)(j >> shift);
-        ent = make(e, j, j & mask);
-        pending = true;
+  for (int64_t base = lo; base < hi; base += (int64_t)nthr * kScatterU) {
+    bool pending[kScatterU];
+    int bk[kScatterU];
+    E ent[kScatterU];
+#pragma unroll
+    for (int u = 0; u < kScatterU; u++) {
+      int64_t e = base + (int64_t)u * nthr + tid;
+      pending[u] = false;
+      bk[u] = 0;
+      int hs = -1;
+      if (e < hi) {
+        uint64_t j;
+        unsigned br = 0, bp = 0;
+        if (decode_index<PACKED>(__builtin_nontemporal_load(idx + e), limit, j, br, bp)) {
+          if (HOT) hs = hot_find(hkeys, (uint32_t)j);
+          if (hs < 0) {
+            bk[u] = (int)(j >> shift);
+            ent[u] = make(e, j, j & mask);
+            pending[u] = true;
+          }
+        }
+      }
+      if (HOT) {
+        if (RUNS) {
+          uint32_t len;
+          int head = wave_runs(hs >= 0 ? (uint32_t)hs : kHotEmpty, len);
+          if (hs >= 0 && lane == head) atomicAdd(&hcnt[hs], len);
+        } else if (hs >= 0) {
+          atomicAdd(&hcnt[hs], 1u);
+        }
       }
     }
-    while (__syncthreads_or(pending)) {
-      if (pending) {
-        uint32_t slot = atomicAdd(&fill[b], 1u);
-        if (slot < (uint32_t)S) {
-          buf[(size_t)b * S + slot] = ent;
-          pending = false;
-          if (slot == (uint32_t)S - 1) list[atomicAdd(nlist, 1u)] = (uint32_t)b;
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < kScatterU; u++) any = any || pending[u];
+    while (__syncthreads_or(any)) {
+#pragma unroll
+      for (int u = 0; u < kScatterU; u++) {
+        uint32_t slot = 0;
+        if (RUNS) {
+          uint32_t len;
+          int head = wave_runs(pending[u] ? (uint32_t)bk[u] : kHotEmpty, len);
+          uint32_t first = 0;
+          if (pending[u] && lane == head) first = atomicAdd(&fill[bk[u]], len);
+          first = __shfl(first, head);
+          slot = first + (uint32_t)(lane - head);
+        } else if (pending[u]) {
+          slot = atomicAdd(&fill[bk[u]], 1u);
+        }
+        if (pending[u]) {
+          if (slot < (uint32_t)S) {
+            buf[(size_t)bk[u] * S + slot] = ent[u];
+            pending[u] = false;
+            if (slot == (uint32_t)S - 1) list[atomicAdd(nlist, 1u)] = (uint32_t)bk[u];
+          }
         }
       }
       __syncthreads();
@@ -175,26 +400,20 @@ __device__ inline void part_scatter_body(const int64_t* __restrict__ idx, int64_
       __syncthreads();
       for (uint32_t k = tid; k < nl; k += nthr) fill[list[k]] = 0;
       if (tid == 0) *nlist = 0;
+      any = false;
+#pragma unroll
+      for (int u = 0; u < kScatterU; u++) any = any || pending[u];
       // the next __syncthreads_or orders these resets before any new append
     }
   }
-  // ship what is left in every buffer
   __syncthreads();
   for (int b = wave; b < B; b += nwaves) {
     uint32_t f = fill[b];
-    if (f) ship(b, f < (uint32_t)S ? f : (uint32_t)S);
+    if (f) ship(b, f);
   }
-}
-
-// LDS bytes the scatter pass needs for B buckets of S entries of `esize` bytes.
-inline size_t part_scatter_lds(int B, int S, size_t esize) {
-  return (((size_t)(2 * B + 1) * 4 + 15) & ~(size_t)15) + (size_t)B * S * esize;
-}
-// Largest power-of-two buffer depth S in [4, 512] that fits `budget` bytes.
-inline int part_pick_depth(int B, size_t esize, size_t budget) {
-  int S = 512;
-  while (S > 4 && part_scatter_lds(B, S, esize) > budget) S >>= 1;
-  return part_scatter_lds(B, S, esize) <= budget ? S : 0;
+  if (HOT)
+    for (int k = tid; k < kHotSlots; k += nthr)
+      if (hcnt[k]) hot_flush(hkeys[k], hcnt[k]);
 }
 
 }  // namespace bale

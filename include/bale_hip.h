@@ -106,6 +106,9 @@ typedef struct bale_config {
   int32_t tile_log2;  /* LDS bin tile / bucket width in log2 bins; 0 = default */
   int32_t merge;      /* BALE_MERGE_*; -1 = default */
   int32_t groups_per_cu; /* resident workgroups per CU for persistent passes; 0 = default */
+  int32_t hot;        /* histogram DIRECT/BUCKET: 1 = count the bins a 4096-index sample
+                         sees twice in a per-workgroup LDS hash table; 0 = off */
+  int32_t task_log2;  /* batches: operations per task (log2, 12..22); 0 = 16 */
 } bale_config;
 
 /* Select the device for this thread's later calls and create its state.
@@ -134,6 +137,16 @@ int bale_sync_errors(void *stream, int64_t *bad);
 int bale_histo_plan(int64_t n, int64_t nbins, int dtype, bale_config *out);
 int bale_gather_plan(int64_t n, int64_t table_len, bale_config *out);
 
+/* ---- device memory for C callers ------------------------------------- */
+/* libgetput/libgetput.h:159 (lgp_all_alloc) at one PE: zeroed device memory
+ * for nblocks * blocksize bytes; NULL on failure.  Free with bale_all_free
+ * (libgetput.h:160, lgp_all_free). */
+void *bale_all_alloc(int64_t nblocks, int64_t blocksize);
+void bale_all_free(void *p);
+/* Host-to-device and device-to-host copies of `bytes` (synchronous). */
+int bale_memput(void *dst_dev, const void *src_host, int64_t bytes);
+int bale_memget(void *dst_host, const void *src_dev, int64_t bytes);
+
 /* ---- input generation on the GPU (seeded, reproducible) ------------- */
 /* table[j] = splitmix64(j) */
 int bale_fill_splitmix(int64_t *table, int64_t n, void *stream);
@@ -146,6 +159,46 @@ int bale_gen_index(int64_t *out, int64_t n, int64_t nbins, int dist, double s,
                    int64_t run_mean, uint64_t seed, void *stream);
 /* pck[i] = index[i] << 16: the packed form at one PE (apps/histo_src/histo.upc:148-150). */
 int bale_pack(int64_t *pck, const int64_t *index, int64_t n, void *stream);
+
+/* ---- batches of uneven instances ------------------------------------- */
+/* No counterpart in the reference (its drivers run one instance per call).
+ * One histogram or gather instance; all pointers are device memory. */
+typedef struct bale_instance {
+  int32_t kind;          /* 0 histogram (counts += ...), 1 gather (tgt = ...) */
+  int32_t dtype;         /* histogram counters: BALE_I64 or BALE_I32 */
+  int64_t n;             /* updates or requests, < 2^32 */
+  int64_t table_len;     /* bins or table entries */
+  const int64_t *index;  /* plain global indices */
+  void *table;           /* counts (histogram) or table (gather) */
+  int64_t *tgt;          /* gather output */
+  bale_config cfg;       /* strategy and task size; BALE_STRAT_AUTO = default rule */
+} bale_instance;
+
+typedef struct bale_batch bale_batch;
+/* Plan `count` instances into per-phase task lists and capture the launch
+ * sequence in one HIP graph.  Instances keep their pointers; relaunching
+ * repeats the work (histograms add again). NULL on error. */
+bale_batch *bale_batch_create(const bale_instance *inst, int64_t count, void *stream);
+/* One graph launch of the whole batch on `stream`. */
+int bale_batch_launch(bale_batch *b, void *stream);
+/* The same kernels enqueued directly (no graph). */
+int bale_batch_launch_direct(bale_batch *b, void *stream);
+/* out[0..5] = phase-A tasks, scatter tasks, apply slots, hot-set sketches,
+ * bucketed instances, scratch bytes. */
+int bale_batch_stats(const bale_batch *b, int64_t *out, int nout);
+void bale_batch_destroy(bale_batch *b);
+/* Baseline: one plain DIRECT kernel launch per instance, in order, on `stream`. */
+int bale_batch_launch_plain(const bale_instance *inst, int64_t count, void *stream);
+
+/* Exhaustive autotuner: time every candidate configuration of each instance
+ * (as a batch of that instance repeated up to min_ops operations, results in
+ * scratch so caller data is untouched) and store the fastest in inst[i].cfg.
+ * costs (optional): count x max_cand seconds per operation, -1 where unused. */
+int bale_tune(bale_instance *inst, int64_t count, int64_t min_ops, float *costs, int max_cand,
+              void *stream);
+/* The candidate list the tuner tries for an instance of this kind and size;
+ * returns its length (writes at most max entries). */
+int bale_tune_candidates(int kind, int64_t table_len, bale_config *out, int max);
 
 /* ---- roofline probes ------------------------------------------------- */
 /* dst[i] = src[i] over `bytes` (multiple of 16): streaming copy rate. */

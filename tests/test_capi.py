@@ -55,7 +55,18 @@ def test_header_compiles_as_c():
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", str(ROOT / "include"), "-x", "c",
                         "-", "-fsyntax-only"], input=src, text=True, capture_output=True)
     assert r.returncode == 0, r.stderr
-    assert C.sizeof(B.BaleConfig) == 24
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    src = ('#include <stdio.h>\n#include <stddef.h>\n#include "bale_hip.h"\n'
+           'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(bale_config), sizeof(bale_instance),'
+           ' offsetof(bale_instance, index), offsetof(bale_instance, cfg));return 0;}\n')
+    (tmp_path / "l.c").write_text(src)
+    exe = tmp_path / "l"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(tmp_path / "l.c"), "-o", str(exe)],
+                   check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert got == [C.sizeof(B.BaleConfig), C.sizeof(B.Instance), B.Instance.index.offset,
+                   B.Instance.cfg.offset]
 
 
 def test_host_side_argument_checks():

@@ -66,8 +66,9 @@ HISTO_CASES = [
     (1 << 20, 1 << 20, 0, 1.0, 64),
     (3_000_017, 1 << 20, 1, 1.1, 0),
 ]
-STRATS = [(B.DIRECT, 0), (B.DIRECT, 1), (B.DIRECT, 2), (B.LDS, 0), (B.LDS, 1), (B.LDS, 2),
-          (B.BUCKET, 0), (B.AUTO, 0)]
+STRATS = [(B.DIRECT, 0, 0), (B.DIRECT, 1, 0), (B.DIRECT, 2, 0), (B.DIRECT, 0, 1),
+          (B.LDS, 0, 0), (B.LDS, 1, 0), (B.LDS, 2, 0),
+          (B.BUCKET, 0, 0), (B.BUCKET, 0, 1), (B.AUTO, 0, 0)]
 
 
 @pytest.mark.parametrize("case", HISTO_CASES, ids=lambda c: f"b{c[0]}_n{c[1]}_d{c[2]}_r{c[4]}")
@@ -79,14 +80,14 @@ def test_histo_strategies_match_oracle(dev, orc, case, dtype):
     base = torch.randint(0, 1000, (nbins,), dtype=dtype, device=dev)
     want = to_np(base).astype(np.int64 if dtype == torch.int64 else np.int32)
     orc.histo_agi(idx_h, want)
-    for strat, merge in STRATS:
+    for strat, merge, hot in STRATS:
         for unroll, block in ((4, 256), (16, 1024)):
             if strat == B.BUCKET and unroll != 4:
                 continue
             counts = base.clone()
-            cfg = B.config(strat, block=block, unroll=unroll, merge=merge)
+            cfg = B.config(strat, block=block, unroll=unroll, merge=merge, hot=hot)
             assert B.histo(idx, counts, cfg=cfg) == 0
-            assert np.array_equal(to_np(counts), want), (strat, merge, unroll, block)
+            assert np.array_equal(to_np(counts), want), (strat, merge, hot, unroll, block)
 
 
 @pytest.mark.parametrize("tile_log2", [9, 12, 15])
@@ -94,10 +95,10 @@ def test_histo_tiles(dev, orc, tile_log2):
     nbins, n = 100_003, 1 << 19
     idx = gen(dev, n, nbins, 1, 1.05, 8, seed=tile_log2)
     want = np.bincount(to_np(idx), minlength=nbins)
-    for strat in (B.LDS, B.BUCKET):
+    for strat, hot in ((B.LDS, 0), (B.BUCKET, 0), (B.BUCKET, 1)):
         counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
-        B.histo(idx, counts, cfg=B.config(strat, block=1024, tile_log2=tile_log2))
-        assert np.array_equal(to_np(counts), want), strat
+        B.histo(idx, counts, cfg=B.config(strat, block=1024, tile_log2=tile_log2, hot=hot))
+        assert np.array_equal(to_np(counts), want), (strat, hot)
 
 
 def test_histo_packed_and_errors(dev, orc):
