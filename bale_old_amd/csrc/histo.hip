@@ -407,7 +407,9 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   if (S == 0) return (int)hipErrorInvalidValue;
   int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
   int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
-  int64_t chunk = ceil_div(n, C);
+  // chunks are whole multiples of kChunkAlign so both passes see the same
+  // aligned 64-update groups
+  int64_t chunk = ceil_div(ceil_div(n, C), kChunkAlign) * kChunkAlign;
   C = ceil_div(n, chunk);
   void* ws = workspace(d, part_bytes(C, B, n, sizeof(uint16_t)), s);
   if (!ws) return (int)hipErrorOutOfMemory;

@@ -27,6 +27,12 @@ constexpr int kHotSlots = 2048;          // LDS hash slots (power of two)
 constexpr int kHotMax = 1024;            // at most this many hot bins
 constexpr uint32_t kHotEmpty = 0xFFFFFFFFu;
 constexpr int kSketch = 4096;            // sampled indices for the hot set
+// In RUNS mode a run of at least this many equal bins within an aligned group
+// of 64 consecutive updates is added with one device atomic.  Both passes see
+// the same groups because chunks are multiples of 64 updates and each wave
+// handles one aligned group per step.
+constexpr uint32_t kLongRun = 4;
+constexpr int64_t kChunkAlign = 4096;
 
 __device__ inline uint32_t hot_hash(uint32_t key) { return (key * 0x9E3779B1u) >> (32 - 11); }
 
@@ -183,6 +189,11 @@ __device__ inline void part_count_body(uint32_t* sh, int64_t c, const int64_t* _
       if (e < hi && decode_index<PACKED>(raw[u], limit, j, bad_r, bad_p))
         if (!HOT || hot_find(hot, (uint32_t)j) < 0) key = (uint32_t)(j >> shift);
       if (RUNS) {
+        // a run of >= kLongRun equal bins is added directly by the scatter
+        // pass and is not partitioned (same aligned 64-element groups there)
+        uint32_t bin = key != kHotEmpty ? (uint32_t)j : kHotEmpty, blen;
+        wave_runs(bin, blen);
+        if (bin != kHotEmpty && blen >= kLongRun) key = kHotEmpty;
         uint32_t len;
         int head = wave_runs(key, len);
         if (key != kHotEmpty && __lane_id() == head) atomicAdd(&hist[key], len);
@@ -291,9 +302,11 @@ inline int part_pick_depth(int B, size_t esize, size_t budget, bool hot) {
 constexpr int kScatterU = 4;  // elements per thread per round
 
 // MakeEntry(e, j, low) -> entry stored for element e with decoded index j
-// (low = j & ((1<<shift)-1)).  HotFlush(bin, count) flushes one hot counter.
+// (low = j & ((1<<shift)-1)).  HotFlush(bin, count) adds count to a bin
+// directly (hot counters at the end, long runs as they are met).
 // RUNS: adjacent lanes with the same bucket (or hot bin) reserve their slots
-// with one LDS atomic per run (clustered inputs).
+// with one LDS atomic per run, and runs of >= kLongRun equal bins skip the
+// partition (clustered inputs).  chunk must be a multiple of 64.
 template <bool PACKED, bool HOT, bool RUNS, typename E, typename MakeEntry, typename HotFlush>
 __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ idx, int64_t n,
                                          uint64_t limit, int shift, int B, int S, int64_t chunk,
@@ -337,27 +350,37 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
     for (uint32_t k = lane; k < cnt; k += 64) staged[pos + k] = buf[(size_t)b * S + k];
   };
 
-  for (int64_t base = lo; base < hi; base += (int64_t)nthr * kScatterU) {
+  const int64_t step = (int64_t)nthr * kScatterU;
+  // indices of the next round are loaded while this round is placed
+  int64_t next[kScatterU];
+#pragma unroll
+  for (int u = 0; u < kScatterU; u++) {
+    int64_t e = lo + (int64_t)u * nthr + tid;
+    next[u] = e < hi ? __builtin_nontemporal_load(idx + e) : 0;
+  }
+  for (int64_t base = lo; base < hi; base += step) {
     bool pending[kScatterU];
     int bk[kScatterU];
     E ent[kScatterU];
+    int64_t raw[kScatterU];
+#pragma unroll
+    for (int u = 0; u < kScatterU; u++) {
+      raw[u] = next[u];
+      int64_t e = base + step + (int64_t)u * nthr + tid;
+      next[u] = e < hi ? __builtin_nontemporal_load(idx + e) : 0;
+    }
 #pragma unroll
     for (int u = 0; u < kScatterU; u++) {
       int64_t e = base + (int64_t)u * nthr + tid;
       pending[u] = false;
       bk[u] = 0;
       int hs = -1;
+      uint64_t j = 0;
+      bool ok = false;
       if (e < hi) {
-        uint64_t j;
         unsigned br = 0, bp = 0;
-        if (decode_index<PACKED>(__builtin_nontemporal_load(idx + e), limit, j, br, bp)) {
-          if (HOT) hs = hot_find(hkeys, (uint32_t)j);
-          if (hs < 0) {
-            bk[u] = (int)(j >> shift);
-            ent[u] = make(e, j, j & mask);
-            pending[u] = true;
-          }
-        }
+        ok = decode_index<PACKED>(raw[u], limit, j, br, bp);
+        if (ok && HOT) hs = hot_find(hkeys, (uint32_t)j);
       }
       if (HOT) {
         if (RUNS) {
@@ -367,6 +390,21 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
         } else if (hs >= 0) {
           atomicAdd(&hcnt[hs], 1u);
         }
+      }
+      bool cold = ok && hs < 0;
+      if (RUNS) {
+        // long runs of one bin are added directly (the count pass skips them)
+        uint32_t len;
+        int head = wave_runs(cold ? (uint32_t)j : kHotEmpty, len);
+        if (cold && len >= kLongRun) {
+          if (lane == head) hot_flush((uint32_t)j, len);
+          cold = false;
+        }
+      }
+      if (cold) {
+        bk[u] = (int)(j >> shift);
+        ent[u] = make(e, j, j & mask);
+        pending[u] = true;
       }
     }
     bool any = false;
