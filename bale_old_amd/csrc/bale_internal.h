@@ -119,5 +119,7 @@ int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbi
 int launch_probe_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 int launch_probe_random(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
                         hipStream_t s);
+int launch_probe_random_policy(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
+                               int policy, hipStream_t s);
 
 }  // namespace bale

@@ -220,7 +220,7 @@ __global__ __launch_bounds__(1024) void k_part_count(const int64_t* __restrict__
 }
 
 // ---- pass 2a: column prefix over chunks (thread per bucket) ----------------------
-static __global__ __launch_bounds__(256) void k_part_offsets(uint32_t* __restrict__ rows, int C,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_part_offsets(uint32_t* __restrict__ rows, int C,
                                                              int B, uint32_t* __restrict__ total) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
@@ -278,7 +278,7 @@ __device__ inline void part_scan_body(const uint32_t* __restrict__ count, int B,
   __syncthreads();
 }
 
-static __global__ __launch_bounds__(1024) void k_part_scan(const uint32_t* __restrict__ count, int B,
+[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_part_scan(const uint32_t* __restrict__ count, int B,
                                                            uint32_t seg_len,
                                                            uint32_t* __restrict__ start,
                                                            uint32_t* __restrict__ segstart,

@@ -1,0 +1,54 @@
+"""One dispatch of each measured kernel, for hardware-counter runs:
+
+    rocprofv3 --pmc FETCH_SIZE -- python3 -m bale_old_amd.prof_workloads gather copy histo
+
+  gather  k_gather_direct over 2^28 uniform requests into a 2^28-entry table
+  copy    k_probe_copy of exactly 1 GiB (read 1 GiB, write 1 GiB): the
+          calibration that converts the counters to bytes
+  histo   the bucketed histogram of 2^28 uniform updates into 2^26 int64 bins
+  zipf    the bucketed histogram with hot bins on Zipf(1.1) updates
+"""
+from __future__ import annotations
+
+import sys
+
+import torch
+
+import bale_old_amd as B
+
+
+def main(argv=None) -> int:
+    which = set(argv if argv is not None else sys.argv[1:]) or {"gather", "copy", "histo"}
+    dev = torch.device("cuda:0")
+    B.check(B.lib().bale_hip_init(0))
+    st = torch.cuda.current_stream()
+    n = 1 << 28
+    idx = torch.empty(n, dtype=torch.int64, device=dev)
+    if which & {"gather", "copy"}:
+        table = torch.empty(n, dtype=torch.int64, device=dev)
+        B.fill_splitmix(table)
+        tgt = torch.empty(n, dtype=torch.int64, device=dev)
+        if "gather" in which:
+            B.gen_index(idx, n, seed=2)
+            B.gather(tgt, idx, table, cfg=B.gather_plan(n, n))
+        if "copy" in which:
+            B.check(B.lib().bale_probe_copy(tgt.data_ptr(), table.data_ptr(), 1 << 30, st.cuda_stream))
+        torch.cuda.synchronize()
+        del table, tgt
+    if "histo" in which:
+        counts = torch.zeros(1 << 26, dtype=torch.int64, device=dev)
+        B.gen_index(idx, 1 << 26, seed=3)
+        B.histo(idx, counts, cfg=B.histo_plan(n, 1 << 26))
+        del counts
+    if "zipf" in which:
+        counts = torch.zeros(1 << 26, dtype=torch.int64, device=dev)
+        B.gen_index(idx, 1 << 26, dist=1, s=1.1, seed=3)
+        B.histo(idx, counts, cfg=B.config(B.BUCKET, block=1024, hot=1))
+        del counts
+    torch.cuda.synchronize()
+    print("prof_workloads done:", " ".join(sorted(which)), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -26,6 +26,17 @@ METRIC = "GUPS for histo & ig at 2^28 updates; learned-vs-plain batched speedup"
 HBM_PEAK_GBS = 8000.0
 
 
+M64 = (1 << 64) - 1
+
+
+def splitmix64(j: int) -> int:
+    """table[j] of the gather workload (first splitmix64 output with state j)."""
+    z = (j + 0x9E3779B97F4A7C15) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -73,12 +84,11 @@ def bench_gather(B, torch, dev, steps, warmup, stream, n_log2=28, t_log2=28):
     total, per = time_steps(step, steps, warmup, stream)
     assert B.sync_errors(stream) == 0
     # size-independent check on a sample: tgt[i] == splitmix64(idx[i])
-    import oracle
     pos = torch.randint(0, n, (4096,), device=dev)
     iv = idx[pos].cpu().tolist()
     tv = tgt[pos].cpu().tolist()
     for a, b in zip(iv, tv):
-        assert (b & 0xFFFFFFFFFFFFFFFF) == oracle.splitmix64(a), "gather mismatch"
+        assert (b & M64) == splitmix64(a), "gather mismatch"
     return {"n": n, "table_len": tlen, "total_s": total, "per_step_s": per, "cfg": cfg.as_dict(),
             "tensors": (table, idx, tgt)}
 
@@ -100,6 +110,99 @@ def bench_histo(B, torch, dev, steps, warmup, stream, n, nbins, dtype, dist, s, 
     assert got == n * (steps + warmup), (got, n * (steps + warmup))
     del idx, counts
     return {"n": n, "nbins": nbins, "total_s": total, "median_s": median(per), "cfg": cfg.as_dict()}
+
+
+def time_calls(fn, reps, stream):
+    """Median seconds of fn() over reps runs (one warm-up), HIP events on stream."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b) * 1e-3)
+    return median(ts)
+
+
+def check_mix_sample(mix, runs, k=48):
+    """Sampled exactness check of a mix after `runs` applications: gathers
+    against splitmix64 of the index, histograms against numpy.bincount."""
+    import numpy as np
+    step = max(1, len(mix.params) // k)
+    for i in range(0, len(mix.params), step):
+        p = mix.params[i]
+        tv, iv, gv = mix.views[i]
+        idx = iv.cpu().numpy()
+        if p.kind == 1:
+            got = gv.cpu().numpy()
+            sel = np.arange(0, idx.size, max(1, idx.size // 512))
+            for s in sel:
+                assert (int(got[s]) & M64) == splitmix64(int(idx[s])), ("gather", i)
+        else:
+            want = np.bincount(idx, minlength=p.table_len) * runs
+            assert np.array_equal(tv.cpu().numpy(), want), ("histo", i)
+
+
+def bench_batch(B, torch, dev, stream, seed, count, reps, min_ops, learned):
+    """The batched uneven mix: plain per-instance launches vs the persistent
+    graph with exhaustive-tuned configurations (and with learned ones)."""
+    from bale_old_amd import mix as MX
+    from bale_old_amd import selector as S
+    params = MX.mix_params(count, seed)
+    mix = MX.Mix(params, dev, stream)
+    torch.cuda.synchronize()
+    ops = mix.total_ops
+    out = {"instances": count, "seed": seed, "ops": ops,
+           "table_bytes": sum(p.table_len for p in params) * 8, "index_bytes": ops * 8}
+    t_plain = time_calls(lambda: MX.launch_plain(mix.inst, count, stream), reps, stream)
+    B.sync_errors(stream)
+    out["plain_ms"] = t_plain * 1e3
+    out["plain_GUPS"] = ops / t_plain / 1e9
+    mix.reset_counts()
+    t0 = time.time()
+    MX.tune(mix.inst, count, min_ops=min_ops, stream=stream)
+    torch.cuda.synchronize()
+    out["tune_seconds"] = time.time() - t0
+    tuned = mix.configs()
+    strat = {}
+    for c in tuned:
+        key = f"{MX.STRATEGIES.get(c.strategy, c.strategy)}{'+hot' if c.hot else ''}{'+runs' if c.merge > 0 else ''}"
+        strat[key] = strat.get(key, 0) + 1
+    out["exhaustive_strategies"] = strat
+    bt = MX.Batch(mix.inst, count, stream)
+    out["batch_stats"] = bt.stats()
+    t_ex = time_calls(lambda: bt.launch(stream), reps, stream)
+    B.sync_errors(stream)
+    check_mix_sample(mix, reps + 1)
+    bt.close()
+    out["exhaustive_ms"] = t_ex * 1e3
+    out["exhaustive_GUPS"] = ops / t_ex / 1e9
+    out["exhaustive_vs_plain"] = t_plain / t_ex
+    if learned is not None:
+        mix.reset_counts()
+        t0 = time.time()
+        S.select(mix.inst, count, learned, dev, stream)
+        torch.cuda.synchronize()
+        out["select_seconds"] = time.time() - t0
+        agree = sum(1 for a, b in zip(tuned, mix.configs())
+                    if S.labels_of(a) == S.labels_of(b))
+        out["learned_agrees_with_exhaustive"] = agree / count
+        bl = MX.Batch(mix.inst, count, stream)
+        t_l = time_calls(lambda: bl.launch(stream), reps, stream)
+        B.sync_errors(stream)
+        check_mix_sample(mix, reps + 1)
+        bl.close()
+        out["learned_ms"] = t_l * 1e3
+        out["learned_GUPS"] = ops / t_l / 1e9
+        out["learned_vs_exhaustive"] = t_ex / t_l
+        out["learned_vs_plain"] = t_plain / t_l
+    del mix
+    torch.cuda.empty_cache()
+    return out
 
 
 def probes(B, torch, dev, stream, table):
@@ -169,6 +272,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="headline gather only")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched mixes")
+    ap.add_argument("--batch-instances", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU exchange is a design (see DESIGN.md), "
@@ -228,6 +333,21 @@ def main() -> int:
             log(f"{name}: {r['GUPS']:.2f} GUPS cfg={r['cfg']}")
             torch.cuda.empty_cache()
         extra["histo"] = histo
+
+    if not args.no_batch:
+        from bale_old_amd import selector as S
+        learned = None
+        if S.WEIGHTS.exists():
+            learned, meta = S.load(dev)
+            extra["selector_training"] = meta
+        log("batched mix, seed 4 (plain vs exhaustive) ...")
+        extra["batch_mix_seed4"] = bench_batch(B, torch, dev, stream, 4, args.batch_instances, 3,
+                                               1 << 20, None)
+        log(f"seed 4: {extra['batch_mix_seed4']}")
+        log("held-out mix, seed 5 (learned vs exhaustive vs plain) ...")
+        extra["batch_mix_seed5"] = bench_batch(B, torch, dev, stream, 5, args.batch_instances, 3,
+                                               1 << 20, learned)
+        log(f"seed 5: {extra['batch_mix_seed5']}")
 
     out = {
         "metric": METRIC,

@@ -207,6 +207,10 @@ int bale_probe_copy(void *dst, const void *src, int64_t bytes, void *stream);
  * read number (no index stream); a digest goes to *out (one int64). */
 int bale_probe_random_read(const int64_t *table, int64_t n, int64_t nreads,
                            int64_t *out, void *stream);
+/* The same through buffer loads with cache policy `policy` (bit 0 sc0, bit 1
+ * nt, bit 4 sc1; one of 0,1,2,3,16,17,18,19); table at most 4 GiB. */
+int bale_probe_random_read_policy(const int64_t *table, int64_t n, int64_t nreads,
+                                  int64_t *out, int policy, void *stream);
 
 #ifdef __cplusplus
 }
