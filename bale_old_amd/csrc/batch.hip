@@ -293,11 +293,14 @@ __global__ __launch_bounds__(1024) void k_batch_c(const BInst* __restrict__ inst
     const BInst& I = insts[T.inst];
     auto make = [](int64_t, uint64_t, uint64_t low) { return (uint16_t)low; };
     auto flush = [&I](uint32_t bin, uint32_t c) { add_count(I, bin, c); };
+    uint16_t* staged = I.staged;
+    auto store = [staged](uint32_t pos, uint16_t v) { staged[pos] = v; };
+    auto skip = [](int64_t) {};
     switch (I.flags & 3) {
-      case 0: part_scatter_body<false, false, false, uint16_t>(T.begin, I.idx, I.n, I.table_len, I.tile_bits, I.B, I.S, I.chunk, I.rows, I.start, I.hot, I.staged, make, flush); break;
-      case 1: part_scatter_body<false, true, false, uint16_t>(T.begin, I.idx, I.n, I.table_len, I.tile_bits, I.B, I.S, I.chunk, I.rows, I.start, I.hot, I.staged, make, flush); break;
-      case 2: part_scatter_body<false, false, true, uint16_t>(T.begin, I.idx, I.n, I.table_len, I.tile_bits, I.B, I.S, I.chunk, I.rows, I.start, I.hot, I.staged, make, flush); break;
-      default: part_scatter_body<false, true, true, uint16_t>(T.begin, I.idx, I.n, I.table_len, I.tile_bits, I.B, I.S, I.chunk, I.rows, I.start, I.hot, I.staged, make, flush); break;
+      case 0: part_scatter_body<false, false, false, uint16_t>(T.begin, I.idx, I.n, I.table_len, I.tile_bits, I.B, I.S, I.chunk, I.rows, I.start, I.hot, store, make, flush, skip); break;
+      case 1: part_scatter_body<false, true, false, uint16_t>(T.begin, I.idx, I.n, I.table_len, I.tile_bits, I.B, I.S, I.chunk, I.rows, I.start, I.hot, store, make, flush, skip); break;
+      case 2: part_scatter_body<false, false, true, uint16_t>(T.begin, I.idx, I.n, I.table_len, I.tile_bits, I.B, I.S, I.chunk, I.rows, I.start, I.hot, store, make, flush, skip); break;
+      default: part_scatter_body<false, true, true, uint16_t>(T.begin, I.idx, I.n, I.table_len, I.tile_bits, I.B, I.S, I.chunk, I.rows, I.start, I.hot, store, make, flush, skip); break;
     }
     __syncthreads();
   }

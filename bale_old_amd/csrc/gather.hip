@@ -77,62 +77,117 @@ __global__ __launch_bounds__(1024) void k_gather_scatter(const int64_t* __restri
                                                          int64_t chunk,
                                                          const uint32_t* __restrict__ rows,
                                                          const uint32_t* __restrict__ start,
-                                                         uint64_t* __restrict__ staged) {
+                                                         uint32_t* __restrict__ off_out,
+                                                         uint32_t* __restrict__ pos_of) {
+  // a shipped entry (slot << 32 | offset) leaves its offset in partition order
+  // and its partition position in pos_of[slot]; a skipped request gets ~0
   part_scatter_body<PACKED, false, false, uint64_t>(
-      blockIdx.x, idx, n, tlen, shift, B, S, chunk, rows, start, nullptr, staged,
+      blockIdx.x, idx, n, tlen, shift, B, S, chunk, rows, start, nullptr,
+      [off_out, pos_of](uint32_t pos, uint64_t ent) {
+        off_out[pos] = (uint32_t)ent;
+        pos_of[ent >> 32] = pos;
+      },
       [](int64_t e, uint64_t, uint64_t low) { return ((uint64_t)e << 32) | low; },
-      [](uint32_t, uint32_t) {});
+      [](uint32_t, uint32_t) {},
+      [pos_of](int64_t e) { pos_of[e] = ~0u; });
 }
 
-// Persistent: a workgroup takes a segment of one region's requests and answers
-// them.  Entries are (request slot << 32) | offset in region.
-__global__ __launch_bounds__(1024) void k_gather_apply(const uint64_t* __restrict__ staged,
-                                                       const uint32_t* __restrict__ start,
-                                                       const uint32_t* __restrict__ segstart,
-                                                       uint32_t* __restrict__ misc, int B,
-                                                       uint32_t seg_len, int shift,
-                                                       const int64_t* __restrict__ table,
-                                                       int64_t* __restrict__ tgt) {
+constexpr int kXcds = 8;  // workgroup b runs on XCD b % 8 (round-robin dispatch)
+
+// Per-XCD segment lists: XCD x serves the regions r with r % 8 == x, in order.
+// xseg[r] = segments of XCD x's regions before r; misc[2 + x] = XCD x's total;
+// misc[10 + x] = XCD x's queue head.  One workgroup, a thread per XCD.
+__global__ __launch_bounds__(64) void k_gather_xseg(const uint32_t* __restrict__ total, int B,
+                                                    uint32_t seg_len, uint32_t* __restrict__ xseg,
+                                                    uint32_t* __restrict__ misc) {
+  const int x = threadIdx.x;
+  if (x >= kXcds) return;
+  uint32_t run = 0;
+  for (int r = x; r < B; r += kXcds) {
+    xseg[r] = run;
+    run += (total[r] + seg_len - 1) / seg_len;
+  }
+  misc[2 + x] = run;
+  misc[10 + x] = 0;
+}
+
+// Persistent, XCD-aware: a workgroup takes segments only from its own XCD's
+// queue, so one region's table lines are read into one XCD's L2 and served
+// from there while that XCD answers the region's requests in partition order.
+__global__ __launch_bounds__(1024) void k_gather_region(const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ xseg,
+                                                        uint32_t* __restrict__ misc, int B,
+                                                        uint32_t seg_len, int shift,
+                                                        const int64_t* __restrict__ table,
+                                                        int64_t* __restrict__ res) {
   __shared__ uint32_t s_work[2];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const uint32_t total = misc[0];
+  const int x = blockIdx.x % kXcds;
+  const uint32_t total = misc[2 + x];
+  const int nr = (B - x + kXcds - 1) / kXcds;  // regions x, x+8, ...
   while (true) {
     if (tid == 0) {
-      uint32_t w = atomicAdd(&misc[1], 1u);
-      uint32_t b = 0;
+      uint32_t w = atomicAdd(&misc[10 + x], 1u);
+      uint32_t r = 0;
       if (w < total) {
-        int lo = 0, hi = B - 1;
+        int lo = 0, hi = nr - 1;  // last region index k with xseg[x + 8k] <= w
         while (lo < hi) {
           int mid = (lo + hi + 1) >> 1;
-          if (segstart[mid] <= w) lo = mid; else hi = mid - 1;
+          if (xseg[x + kXcds * mid] <= w) lo = mid; else hi = mid - 1;
         }
-        b = (uint32_t)lo;
+        r = (uint32_t)(x + kXcds * lo);
       }
       s_work[0] = w;
-      s_work[1] = b;
+      s_work[1] = r;
     }
     __syncthreads();
-    const uint32_t w = s_work[0], b = s_work[1];
+    const uint32_t w = s_work[0], r = s_work[1];
+    __syncthreads();
     if (w >= total) break;
-    const uint32_t s = w - segstart[b];
-    const uint32_t e0 = start[b] + s * seg_len;
-    const uint32_t e1 = min(start[b + 1], e0 + seg_len);
-    const int64_t* region = table + ((int64_t)b << shift);
-    for (uint32_t k = e0 + tid; k < e1; k += 4 * nthr) {
-      uint64_t ent[4];
-      int64_t v[4];
+    const uint32_t e0 = start[r] + (w - xseg[r]) * seg_len;
+    const uint32_t e1 = min(start[r + 1], e0 + seg_len);
+    const int64_t* region = table + ((int64_t)r << shift);
+    constexpr int Q = 8;
+    for (uint32_t k = e0 + tid; k < e1; k += Q * nthr) {
+      uint32_t o[Q];
+      int64_t v[Q];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < Q; q++) {
         uint32_t kk = k + q * nthr;
-        ent[q] = kk < e1 ? __builtin_nontemporal_load(staged + kk) : ~0ULL;
+        o[q] = kk < e1 ? __builtin_nontemporal_load(off + kk) : 0u;
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++) v[q] = ent[q] != ~0ULL ? region[(uint32_t)ent[q]] : 0;
+      for (int q = 0; q < Q; q++) v[q] = region[o[q]];
 #pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (ent[q] != ~0ULL) tgt[ent[q] >> 32] = v[q];
+      for (int q = 0; q < Q; q++) {
+        uint32_t kk = k + q * nthr;
+        if (kk < e1) __builtin_nontemporal_store(v[q], res + kk);
+      }
     }
-    __syncthreads();
+  }
+}
+
+// tgt[i] = res[pos_of[i]]: a request's reply sits where its request was staged.
+// pos_of of consecutive requests walks a few monotone ranges of res, so the
+// reads stay in cache.  A skipped request (~0) gets 0.
+__global__ __launch_bounds__(256) void k_gather_unpermute(const uint32_t* __restrict__ pos_of,
+                                                          const int64_t* __restrict__ res,
+                                                          const uint32_t* __restrict__ start,
+                                                          int B, int64_t n,
+                                                          int64_t* __restrict__ tgt) {
+  const uint32_t valid = start[B];
+  int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
+  for (int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; i0 < n; i0 += stride) {
+    uint32_t p[4];
+    int64_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) p[q] = i0 + q < n ? __builtin_nontemporal_load(pos_of + i0 + q) : ~0u;
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = p[q] < valid ? res[p[q]] : 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (i0 + q < n) __builtin_nontemporal_store(v[q], tgt + i0 + q);
   }
 }
 
@@ -192,9 +247,15 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
   int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
   int64_t chunk = ceil_div(n, C);
   C = ceil_div(n, chunk);
-  void* ws = workspace(d, part_bytes(C, B, n, sizeof(uint64_t)), s);
+  // staging: offsets in partition order (u32), each request's partition
+  // position (u32), and the replies in partition order (i64)
+  const size_t half = (((size_t)n * 4) + 255) & ~(size_t)255;
+  void* ws = workspace(d, part_bytes(C, B, 0, 1) + 2 * half + (size_t)n * 8 + 256, s);
   if (!ws) return (int)hipErrorOutOfMemory;
   PartLayout L = part_carve(ws, C, B);
+  uint32_t* off = (uint32_t*)L.staged;
+  uint32_t* pos_of = (uint32_t*)((char*)L.staged + half);
+  int64_t* res = (int64_t*)((char*)L.staged + 2 * half);
   size_t lds_count = (size_t)B * 4;
   if (packed)
     hipLaunchKernelGGL((k_part_count<true, false, false>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
@@ -204,24 +265,29 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
                        n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);
-  int64_t seg = std::max<int64_t>(4096, ceil_div(n, (int64_t)d->cus * 4));
+  // segments: a few per region so an XCD's workgroups share a region at a time
+  int64_t seg = std::max<int64_t>(8192, ceil_div(n, (int64_t)d->cus * 8));
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
                      L.start, L.segstart, L.misc);
+  uint32_t* xseg = L.segstart;  // reused: per-XCD segment starts
+  hipLaunchKernelGGL(k_gather_xseg, dim3(1), dim3(64), 0, s, L.total, (int)B, (uint32_t)seg, xseg,
+                     L.misc);
   size_t lds = part_scatter_lds((int)B, S, sizeof(uint64_t), false);
   if (packed) {
     allow_lds(k_gather_scatter<true>, lds);
     hipLaunchKernelGGL(k_gather_scatter<true>, dim3((unsigned)C), dim3(1024), lds, s, idx, n,
-                       (uint64_t)table_len, shift, (int)B, S, chunk, L.chunk, L.start,
-                       (uint64_t*)L.staged);
+                       (uint64_t)table_len, shift, (int)B, S, chunk, L.chunk, L.start, off, pos_of);
   } else {
     allow_lds(k_gather_scatter<false>, lds);
     hipLaunchKernelGGL(k_gather_scatter<false>, dim3((unsigned)C), dim3(1024), lds, s, idx, n,
-                       (uint64_t)table_len, shift, (int)B, S, chunk, L.chunk, L.start,
-                       (uint64_t*)L.staged);
+                       (uint64_t)table_len, shift, (int)B, S, chunk, L.chunk, L.start, off, pos_of);
   }
-  hipLaunchKernelGGL(k_gather_apply, dim3((unsigned)(d->cus * 2)), dim3(1024), 0, s,
-                     (const uint64_t*)L.staged, L.start, L.segstart, L.misc, (int)B, (uint32_t)seg,
-                     shift, table, tgt);
+  int gr = c.groups_per_cu > 0 ? c.groups_per_cu : 2;
+  hipLaunchKernelGGL(k_gather_region, dim3((unsigned)(d->cus * gr)), dim3(1024), 0, s, off, L.start,
+                     xseg, L.misc, (int)B, (uint32_t)seg, shift, table, res);
+  int64_t ug = std::min<int64_t>(ceil_div(n, 1024), (int64_t)d->cus * 16);
+  hipLaunchKernelGGL(k_gather_unpermute, dim3((unsigned)ug), dim3(256), 0, s, pos_of, res, L.start,
+                     (int)B, n, tgt);
   return (int)hipGetLastError();
 }
 

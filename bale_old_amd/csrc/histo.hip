@@ -176,9 +176,11 @@ __global__ __launch_bounds__(1024) void k_histo_scatter(const int64_t* __restric
                                                         uint16_t* __restrict__ staged,
                                                         CT* __restrict__ counts) {
   part_scatter_body<PACKED, HOT, RUNS, uint16_t>(
-      blockIdx.x, idx, n, nbins, shift, B, S, chunk, rows, start, hot_g, staged,
+      blockIdx.x, idx, n, nbins, shift, B, S, chunk, rows, start, hot_g,
+      [staged](uint32_t pos, uint16_t v) { staged[pos] = v; },
       [](int64_t, uint64_t, uint64_t low) { return (uint16_t)low; },
-      [counts](uint32_t bin, uint32_t c) { atomic_add_dev(counts + bin, (CT)c); });
+      [counts](uint32_t bin, uint32_t c) { atomic_add_dev(counts + bin, (CT)c); },
+      [](int64_t) {});
 }
 
 // Add the 8 16-bit bin offsets packed in x to the LDS histogram.  RUNS: equal

@@ -48,19 +48,22 @@ __device__ inline int hot_find(const uint32_t* keys, uint32_t key) {
   return -1;
 }
 
-// Workspace layout (32-bit words, then the staging array).
+// Workspace layout (32-bit words, then the staging array).  misc: [0] total
+// apply segments, [1] apply queue head, [2..9] per-XCD segment totals,
+// [10..17] per-XCD queue heads.
+constexpr int kMiscWords = 32;
 struct PartLayout {
   uint32_t* chunk;     // [C*B] per-chunk counts, then per-chunk first slots
   uint32_t* start;     // [B+1] bucket starts
   uint32_t* total;     // [B]   items per bucket
   uint32_t* segstart;  // [B+1] first apply segment of each bucket
-  uint32_t* misc;      // [4]   [0] total segments, [1] work-queue head
+  uint32_t* misc;      // [kMiscWords] queue totals and heads
   uint32_t* hot;       // [kHotSlots] hot-bin hash table (keys)
   void* staged;        // [n]   entries grouped by bucket
 };
 
 inline size_t part_words(int64_t C, int64_t B) {
-  return (size_t)(C * B + (B + 1) + B + (B + 1) + 4 + kHotSlots);
+  return (size_t)(C * B + (B + 1) + B + (B + 1) + kMiscWords + kHotSlots);
 }
 inline size_t part_bytes(int64_t C, int64_t B, int64_t n, size_t esize) {
   size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
@@ -74,7 +77,7 @@ inline PartLayout part_carve(void* ws, int64_t C, int64_t B) {
   L.total = L.start + (B + 1);
   L.segstart = L.total + B;
   L.misc = L.segstart + (B + 1);
-  L.hot = L.misc + 4;
+  L.hot = L.misc + kMiscWords;
   size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
   L.staged = (char*)ws + head;
   return L;
@@ -287,9 +290,9 @@ __device__ inline void part_scan_body(const uint32_t* __restrict__ count, int B,
 }
 
 // ---- pass 3: scatter through LDS send buffers -------------------------------------
-// LDS layout: cursor[B], fill[B], list[B], nlist, [hot keys, hot counts], buf[B*S].
+// LDS layout: cursor[B], fill[B], list[B], nlist[2], [hot keys, hot counts], buf[B*S].
 inline size_t part_scatter_lds(int B, int S, size_t esize, bool hot) {
-  size_t head = (size_t)(3 * B + 1) * 4 + (hot ? (size_t)2 * kHotSlots * 4 : 0);
+  size_t head = (size_t)(3 * B + 2) * 4 + (hot ? (size_t)2 * kHotSlots * 4 : 0);
   return ((head + 15) & ~(size_t)15) + (size_t)B * S * esize;
 }
 // Largest power-of-two buffer depth S in [4, 256] that fits `budget` bytes (0 if none).
@@ -307,22 +310,24 @@ constexpr int kScatterU = 4;  // elements per thread per round
 // RUNS: adjacent lanes with the same bucket (or hot bin) reserve their slots
 // with one LDS atomic per run, and runs of >= kLongRun equal bins skip the
 // partition (clustered inputs).  chunk must be a multiple of 64.
-template <bool PACKED, bool HOT, bool RUNS, typename E, typename MakeEntry, typename HotFlush>
+// Store(pos, entry) writes one shipped entry to staging position pos;
+// Skip(e) is called for an element whose index is out of range.
+template <bool PACKED, bool HOT, bool RUNS, typename E, typename MakeEntry, typename HotFlush,
+          typename Store, typename Skip>
 __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ idx, int64_t n,
                                          uint64_t limit, int shift, int B, int S, int64_t chunk,
                                          const uint32_t* __restrict__ rows,
                                          const uint32_t* __restrict__ start,
-                                         const uint32_t* __restrict__ hot_g,
-                                         E* __restrict__ staged, MakeEntry make,
-                                         HotFlush hot_flush) {
+                                         const uint32_t* __restrict__ hot_g, Store store,
+                                         MakeEntry make, HotFlush hot_flush, Skip skip) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* cursor = (uint32_t*)smem;  // [B] next staging slot of this chunk per bucket
   uint32_t* fill = cursor + B;         // [B]
   uint32_t* list = fill + B;           // [B] buckets to ship this step
-  uint32_t* nlist = list + B;          // [1]
-  uint32_t* hkeys = nlist + 1;         // [kHotSlots]
+  uint32_t* nlist = list + B;          // [2] list lengths, alternating between steps
+  uint32_t* hkeys = nlist + 2;         // [kHotSlots]
   uint32_t* hcnt = hkeys + kHotSlots;  // [kHotSlots]
-  size_t head = (size_t)(3 * B + 1) * 4 + (HOT ? (size_t)2 * kHotSlots * 4 : 0);
+  size_t head = (size_t)(3 * B + 2) * 4 + (HOT ? (size_t)2 * kHotSlots * 4 : 0);
   E* buf = (E*)(smem + ((head + 15) & ~(size_t)15));  // [B*S]
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = __lane_id(), wave = tid >> 6, nwaves = nthr >> 6;
@@ -336,18 +341,51 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
       hkeys[k] = hot_g[k];
       hcnt[k] = 0;
     }
-  if (tid == 0) *nlist = 0;
+  if (tid == 0) nlist[0] = nlist[1] = 0;
   __syncthreads();
   const uint64_t mask = (1ULL << shift) - 1;
   int64_t lo = c * chunk;
   int64_t hi = lo + chunk < n ? lo + chunk : n;
+  uint32_t p = 0;  // which list length this step appends to
+
+  // Ship every full buffer listed this step: wave w takes list entries
+  // w, w + nwaves, ...; its lanes reserve the entries' output ranges at once
+  // (a bucket is listed at most once per step, so its cursor has one owner),
+  // then copy 64/S buffers per instruction.
+  auto ship_listed = [&](uint32_t nl) {
+    for (uint32_t g0 = wave; g0 < nl; g0 += (uint32_t)nwaves * 64u) {
+      uint32_t k = g0 + (uint32_t)lane * nwaves;
+      uint32_t b_l = 0, pos_l = 0;
+      if (k < nl) {
+        b_l = list[k];
+        pos_l = cursor[b_l];
+        cursor[b_l] = pos_l + (uint32_t)S;
+        fill[b_l] = 0;
+      }
+      const uint32_t cnt = min(64u, (nl - g0 + nwaves - 1) / (uint32_t)nwaves);
+      if (S <= 64) {
+        const uint32_t per = 64u / (uint32_t)S;
+        const uint32_t q = (uint32_t)lane % (uint32_t)S;
+        for (uint32_t m0 = 0; m0 < cnt; m0 += per) {
+          uint32_t m = m0 + (uint32_t)lane / (uint32_t)S;
+          uint32_t b = __shfl(b_l, (int)(m & 63u)), pos = __shfl(pos_l, (int)(m & 63u));
+          if (m < cnt) store(pos + q, buf[(size_t)b * S + q]);
+        }
+      } else {
+        for (uint32_t m = 0; m < cnt; m++) {
+          uint32_t b = __shfl(b_l, (int)m), pos = __shfl(pos_l, (int)m);
+          for (uint32_t q = lane; q < (uint32_t)S; q += 64) store(pos + q, buf[(size_t)b * S + q]);
+        }
+      }
+    }
+  };
 
   // one wave ships the first `cnt` entries of bucket b to the chunk's range
   auto ship = [&](int b, uint32_t cnt) {
     uint32_t pos = 0;
     if (lane == 0) pos = atomicAdd(&cursor[b], cnt);  // LDS atomic
     pos = __shfl(pos, 0);
-    for (uint32_t k = lane; k < cnt; k += 64) staged[pos + k] = buf[(size_t)b * S + k];
+    for (uint32_t k = lane; k < cnt; k += 64) store(pos + k, buf[(size_t)b * S + k]);
   };
 
   const int64_t step = (int64_t)nthr * kScatterU;
@@ -381,6 +419,7 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
         unsigned br = 0, bp = 0;
         ok = decode_index<PACKED>(raw[u], limit, j, br, bp);
         if (ok && HOT) hs = hot_find(hkeys, (uint32_t)j);
+        if (!ok) skip(e);
       }
       if (HOT) {
         if (RUNS) {
@@ -428,20 +467,20 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
           if (slot < (uint32_t)S) {
             buf[(size_t)bk[u] * S + slot] = ent[u];
             pending[u] = false;
-            if (slot == (uint32_t)S - 1) list[atomicAdd(nlist, 1u)] = (uint32_t)bk[u];
+            if (slot == (uint32_t)S - 1) list[atomicAdd(&nlist[p], 1u)] = (uint32_t)bk[u];
           }
         }
       }
       __syncthreads();
-      uint32_t nl = *nlist;
-      for (uint32_t k = wave; k < nl; k += nwaves) ship((int)list[k], (uint32_t)S);
-      __syncthreads();
-      for (uint32_t k = tid; k < nl; k += nthr) fill[list[k]] = 0;
-      if (tid == 0) *nlist = 0;
+      // ships reset fill[] of what they ship; the other list length (last used
+      // a step ago, whose readers are past the barrier above) is cleared here
+      ship_listed(nlist[p]);
+      if (tid == 0) nlist[p ^ 1] = 0;
+      p ^= 1;
       any = false;
 #pragma unroll
       for (int u = 0; u < kScatterU; u++) any = any || pending[u];
-      // the next __syncthreads_or orders these resets before any new append
+      // the next __syncthreads_or orders the ships and resets before any new append
     }
   }
   __syncthreads();
