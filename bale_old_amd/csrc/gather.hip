@@ -169,25 +169,32 @@ __global__ __launch_bounds__(1024) void k_gather_region(const uint32_t* __restri
 }
 
 // tgt[i] = res[pos_of[i]]: a request's reply sits where its request was staged.
-// pos_of of consecutive requests walks a few monotone ranges of res, so the
-// reads stay in cache.  A skipped request (~0) gets 0.
-__global__ __launch_bounds__(256) void k_gather_unpermute(const uint32_t* __restrict__ pos_of,
-                                                          const int64_t* __restrict__ res,
-                                                          const uint32_t* __restrict__ start,
-                                                          int B, int64_t n,
-                                                          int64_t* __restrict__ tgt) {
+// One workgroup streams one partition chunk: that chunk's replies fill one
+// range per region, each consumed front to back, so the reads stay in cache.
+// A skipped request (~0) gets 0.
+__global__ __launch_bounds__(1024) void k_gather_unpermute(const uint32_t* __restrict__ pos_of,
+                                                           const int64_t* __restrict__ res,
+                                                           const uint32_t* __restrict__ start,
+                                                           int B, int64_t n, int64_t chunk,
+                                                           int64_t* __restrict__ tgt) {
   const uint32_t valid = start[B];
-  int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
-  for (int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; i0 < n; i0 += stride) {
-    uint32_t p[4];
-    int64_t v[4];
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(n, lo + chunk);
+  constexpr int Q = 4;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)blockDim.x * Q) {
+    uint32_t p[Q];
+    int64_t v[Q];
 #pragma unroll
-    for (int q = 0; q < 4; q++) p[q] = i0 + q < n ? __builtin_nontemporal_load(pos_of + i0 + q) : ~0u;
+    for (int q = 0; q < Q; q++) {
+      int64_t i = i0 + (int64_t)q * blockDim.x;
+      p[q] = i < hi ? __builtin_nontemporal_load(pos_of + i) : ~0u;
+    }
 #pragma unroll
-    for (int q = 0; q < 4; q++) v[q] = p[q] < valid ? res[p[q]] : 0;
+    for (int q = 0; q < Q; q++) v[q] = p[q] < valid ? res[p[q]] : 0;
 #pragma unroll
-    for (int q = 0; q < 4; q++)
-      if (i0 + q < n) __builtin_nontemporal_store(v[q], tgt + i0 + q);
+    for (int q = 0; q < Q; q++) {
+      int64_t i = i0 + (int64_t)q * blockDim.x;
+      if (i < hi) __builtin_nontemporal_store(v[q], tgt + i);
+    }
   }
 }
 
@@ -265,8 +272,11 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
                        n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);
-  // segments: a few per region so an XCD's workgroups share a region at a time
-  int64_t seg = std::max<int64_t>(8192, ceil_div(n, (int64_t)d->cus * 8));
+  // segments: one region's requests split over all of an XCD's workgroups,
+  // so they work on the same (L2-resident) region at a time
+  int gr = c.groups_per_cu > 0 ? c.groups_per_cu : 2;
+  const int64_t per_xcd = std::max<int64_t>(1, (int64_t)d->cus * gr / kXcds);
+  int64_t seg = std::max<int64_t>(2048, ceil_div(ceil_div(n, B), per_xcd));
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
                      L.start, L.segstart, L.misc);
   uint32_t* xseg = L.segstart;  // reused: per-XCD segment starts
@@ -282,12 +292,10 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
     hipLaunchKernelGGL(k_gather_scatter<false>, dim3((unsigned)C), dim3(1024), lds, s, idx, n,
                        (uint64_t)table_len, shift, (int)B, S, chunk, L.chunk, L.start, off, pos_of);
   }
-  int gr = c.groups_per_cu > 0 ? c.groups_per_cu : 2;
   hipLaunchKernelGGL(k_gather_region, dim3((unsigned)(d->cus * gr)), dim3(1024), 0, s, off, L.start,
                      xseg, L.misc, (int)B, (uint32_t)seg, shift, table, res);
-  int64_t ug = std::min<int64_t>(ceil_div(n, 1024), (int64_t)d->cus * 16);
-  hipLaunchKernelGGL(k_gather_unpermute, dim3((unsigned)ug), dim3(256), 0, s, pos_of, res, L.start,
-                     (int)B, n, tgt);
+  hipLaunchKernelGGL(k_gather_unpermute, dim3((unsigned)C), dim3(1024), 0, s, pos_of, res, L.start,
+                     (int)B, n, chunk, tgt);
   return (int)hipGetLastError();
 }
 
