@@ -99,13 +99,18 @@ static void plan_histo(const Device* d, int64_t n, int64_t nbins, int dtype, bal
   c->merge = BALE_MERGE_NONE;
   int tb = max_tile_bits_for(d);
   int64_t bytes = nbins * (dtype == BALE_I64 ? 8 : 4);
+  (void)bytes;
   if (nbins <= ((int64_t)1 << tb) && n >= 8 * nbins) {
     c->strategy = BALE_STRAT_LDS;
     c->block = 1024;
     c->unroll = 8;
-  } else if (bytes > ((int64_t)64 << 20) && (nbins >> tb) <= 4096 && n >= nbins / 4) {
+  } else if (n >= ((int64_t)1 << 20) && ceil_div(nbins, (int64_t)1 << tb) <= 4096 &&
+             n >= nbins / 4) {
+    // partition through LDS; the sample decides about hot bins and runs
     c->strategy = BALE_STRAT_BUCKET;
     c->block = 1024;
+    c->hot = 2;
+    c->tile_log2 = nbins <= ((int64_t)1 << 22) ? 14 : 0;
   } else {
     c->strategy = BALE_STRAT_DIRECT;
     c->block = 256;

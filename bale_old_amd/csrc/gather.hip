@@ -266,11 +266,13 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
   size_t lds_count = (size_t)B * 4;
   if (packed)
     hipLaunchKernelGGL((k_part_count<true, false, false>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
-                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err);
+                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err,
+                       (const uint32_t*)nullptr);
   else
     hipLaunchKernelGGL((k_part_count<false, false, false>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
-                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err);
-  hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, s, L.chunk,
+                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err,
+                       (const uint32_t*)nullptr);
+  hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);
   // segments: one region's requests split over all of an XCD's workgroups,
   // so they work on the same (L2-resident) region at a time

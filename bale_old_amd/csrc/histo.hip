@@ -174,13 +174,15 @@ __global__ __launch_bounds__(1024) void k_histo_scatter(const int64_t* __restric
                                                         const uint32_t* __restrict__ start,
                                                         const uint32_t* __restrict__ hot_g,
                                                         uint16_t* __restrict__ staged,
-                                                        CT* __restrict__ counts) {
-  part_scatter_body<PACKED, HOT, RUNS, uint16_t>(
-      blockIdx.x, idx, n, nbins, shift, B, S, chunk, rows, start, hot_g,
+                                                        CT* __restrict__ counts,
+                                                        const uint32_t* __restrict__ flags,
+                                                        int S_cold) {
+  part_scatter_adaptive<PACKED, HOT, RUNS, uint16_t>(
+      blockIdx.x, idx, n, nbins, shift, B, S, S_cold, chunk, rows, start, hot_g,
       [staged](uint32_t pos, uint16_t v) { staged[pos] = v; },
       [](int64_t, uint64_t, uint64_t low) { return (uint16_t)low; },
       [counts](uint32_t bin, uint32_t c) { atomic_add_dev(counts + bin, (CT)c); },
-      [](int64_t) {});
+      [](int64_t) {}, flags);
 }
 
 // Add the 8 16-bit bin offsets packed in x to the LDS histogram.  RUNS: equal
@@ -217,9 +219,11 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
                                                       const uint32_t* __restrict__ segstart,
                                                       uint32_t* __restrict__ misc, int B,
                                                       uint32_t seg_len, int shift, uint64_t nbins,
-                                                      CT* __restrict__ counts) {
+                                                      CT* __restrict__ counts,
+                                                      const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
   __shared__ uint32_t s_work[4];
+  const bool merge = RUNS && (!flags || (*flags & kFlagRunsSeen));
   const int tid = threadIdx.x, nthr = blockDim.x;
   const uint32_t total = misc[0];
   while (true) {
@@ -255,7 +259,9 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
     uint32_t nvec = (e1 - a0) / 8u;
     const u32x4* v = (const u32x4*)(staged + a0);
     for (uint32_t q = tid; q < nvec; q += nthr) {
-      lds_add8<RUNS>(sh, __builtin_nontemporal_load(v + q));
+      u32x4 x = __builtin_nontemporal_load(v + q);
+      if (merge) lds_add8<true>(sh, x);
+      else lds_add8<false>(sh, x);
     }
     for (uint32_t k = a0 + nvec * 8u + tid; k < e1; k += nthr) atomicAdd(&sh[staged[k]], 1u);
     __syncthreads();
@@ -309,7 +315,8 @@ static int direct_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
     void* ws = workspace(d, kHotSlots * 4, s);
     if (!ws) return (int)hipErrorOutOfMemory;
     uint32_t* hot = (uint32_t*)ws;
-    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins, hot);
+    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins, hot,
+                       (uint32_t*)nullptr);
     int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : (block == 1024 ? 2 : 4);
     grid = std::min<int64_t>(grid, (int64_t)d->cus * gpc);
     dim3 g((unsigned)grid), b(block);
@@ -416,12 +423,17 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   void* ws = workspace(d, part_bytes(C, B, n, sizeof(uint16_t)), s);
   if (!ws) return (int)hipErrorOutOfMemory;
   PartLayout L = part_carve(ws, C, B);
+  // hot == 2: the sketch decides on the device whether hot bins / runs are
+  // worth handling; every pass reads its verdict from the flag word
+  uint32_t* flags = c.hot == 2 ? L.misc + (kMiscWords - 1) : nullptr;
   if (HOT)
-    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins, L.hot);
+    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins, L.hot,
+                       flags);
   size_t lds_count = (size_t)(B + (HOT ? kHotSlots : 0)) * 4;
   hipLaunchKernelGGL((k_part_count<PACKED, HOT, RUNS>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
-                     n, (uint64_t)nbins, tb, (int)B, chunk, L.chunk, L.hot, d->err);
-  hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, s, L.chunk,
+                     n, (uint64_t)nbins, tb, (int)B, chunk, L.chunk, L.hot, d->err,
+                     (const uint32_t*)flags);
+  hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);
   // segments: enough of them to spread the apply pass over the chip
   // Long enough that a segment's flush (up to one add per bin of the tile) stays
@@ -430,23 +442,30 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
                      L.start, L.segstart, L.misc);
   size_t lds = part_scatter_lds((int)B, S, sizeof(uint16_t), HOT);
+  // adaptive mode: when the sample finds no hot bins, the scatter pass puts
+  // its buffers over the hot table, as deep as the no-hot layout allows
+  int S_cold = 0;
+  if (HOT && flags) {
+    S_cold = part_pick_depth((int)B, sizeof(uint16_t), budget, false);
+    lds = std::max(lds, part_scatter_lds((int)B, S_cold, sizeof(uint16_t), false));
+  }
   allow_lds(k_histo_scatter<PACKED, HOT, RUNS, CT>, lds);
   hipLaunchKernelGGL((k_histo_scatter<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024), lds, s, idx,
                      n, (uint64_t)nbins, tb, (int)B, S, chunk, L.chunk, L.start, L.hot,
-                     (uint16_t*)L.staged, counts);
+                     (uint16_t*)L.staged, counts, (const uint32_t*)flags, S_cold);
   size_t lds_apply = (size_t)4 << tb;
   int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
   allow_lds(k_histo_apply<CT, RUNS>, lds_apply);
   hipLaunchKernelGGL((k_histo_apply<CT, RUNS>), dim3((unsigned)groups), dim3(1024), lds_apply, s,
                      (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B, (uint32_t)seg,
-                     tb, (uint64_t)nbins, counts);
+                     tb, (uint64_t)nbins, counts, (const uint32_t*)flags);
   return (int)hipGetLastError();
 }
 
 template <typename CT>
 static int bucket_dispatch(const int64_t* idx, int64_t n, CT* counts, int64_t nbins, bool packed,
                            const bale_config& c, Device* d, hipStream_t s) {
-  const bool hot = c.hot != 0, runs = c.merge > 0;
+  const bool hot = c.hot != 0, runs = c.merge > 0 || c.hot == 2;
   const int sel = (packed ? 4 : 0) | (hot ? 2 : 0) | (runs ? 1 : 0);
   switch (sel) {
     case 0: return bucket_typed<false, false, false, CT>(idx, n, counts, nbins, c, d, s);

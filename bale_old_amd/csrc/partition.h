@@ -84,61 +84,95 @@ inline PartLayout part_carve(void* ws, int64_t C, int64_t B) {
 }
 
 // ---- hot-set sketch (one workgroup of 1024) -----------------------------------
-// Samples kSketch indices at a fixed stride, sorts them in LDS and writes the
-// hash table of bins that occur at least twice (at most kHotMax of them).
+// Samples kSketch indices at a fixed stride, counts them in an LDS hash table
+// (2 * kSketch slots, so probing stays short and always finds a slot) and
+// writes the hash table of bins that occur at least twice (at most kHotMax).
+// With `flags`, also writes the sample's verdict for adaptive passes:
+// bit kFlagHotBins when the repeated bins carry enough of the sample to be
+// worth a hot table, bit kFlagRunsSeen when at least 1/8 of the sampled
+// positions equal their right neighbour (clustered runs).
+constexpr uint32_t kFlagHotBins = 1, kFlagRunsSeen = 2;
+constexpr int kSketchSlots = 2 * kSketch;
+
+__device__ inline uint32_t sketch_hash(uint32_t v) {
+  return (v * 0x9E3779B1u) >> (32 - 13);  // kSketchSlots = 2^13
+}
+static_assert(kSketchSlots == (1 << 13), "sketch_hash assumes 2^13 slots");
+
 template <bool PACKED>
 __device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t n, uint64_t limit,
-                                       uint32_t* __restrict__ hot_out) {
-  __shared__ uint32_t s[kSketch];
+                                       uint32_t* __restrict__ hot_out,
+                                       uint32_t* __restrict__ flags = nullptr) {
+  __shared__ uint32_t keys[kSketchSlots];
+  __shared__ uint32_t cnt2[kSketchSlots / 2];  // two 16-bit counts per word
   __shared__ uint32_t table[kHotSlots];
-  __shared__ uint32_t nhot;
-  const int tid = threadIdx.x;
-  for (int k = tid; k < kSketch; k += 1024) {
+  __shared__ uint32_t nhot, neq, npairs, ndup;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int k = tid; k < kSketchSlots; k += nthr) keys[k] = kHotEmpty;
+  for (int k = tid; k < kSketchSlots / 2; k += nthr) cnt2[k] = 0;
+  for (int k = tid; k < kHotSlots; k += nthr) table[k] = kHotEmpty;
+  if (tid == 0) nhot = neq = npairs = ndup = 0;
+  __syncthreads();
+  uint32_t eq = 0, pairs = 0;
+  for (int k = tid; k < kSketch; k += nthr) {
     int64_t pos = (int64_t)(((unsigned __int128)k * (uint64_t)n) / kSketch);
-    uint64_t j;
+    // both loads issued before either is used
+    const int64_t r0 = pos < n ? idx[pos] : 0;
+    const int64_t r1 = flags && pos + 1 < n ? idx[pos + 1] : 0;
+    uint64_t j = 0, j2 = 0;
     unsigned br = 0, bp = 0;
-    s[k] = (pos < n && decode_index<PACKED>(idx[pos], limit, j, br, bp) && j < kHotEmpty)
-               ? (uint32_t)j : kHotEmpty;
-  }
-  for (int k = tid; k < kHotSlots; k += 1024) table[k] = kHotEmpty;
-  if (tid == 0) nhot = 0;
-  __syncthreads();
-  // bitonic sort of s[0..kSketch)
-  for (int size = 2; size <= kSketch; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int k = tid; k < kSketch / 2; k += 1024) {
-        int lo = 2 * k - (k & (stride - 1));
-        int hi = lo + stride;
-        bool up = (lo & size) == 0;
-        uint32_t a = s[lo], b = s[hi];
-        if ((a > b) == up) {
-          s[lo] = b;
-          s[hi] = a;
-        }
+    bool ok = pos < n && decode_index<PACKED>(r0, limit, j, br, bp);
+    if (ok && j < kHotEmpty) {
+      // at most kSketch keys in 2 * kSketch slots: the probe always ends
+      uint32_t v = (uint32_t)j, h = sketch_hash(v);
+      for (;;) {
+        uint32_t prev = atomicCAS(&keys[h], kHotEmpty, v);
+        if (prev == kHotEmpty || prev == v) break;
+        h = (h + 1) & (kSketchSlots - 1);
       }
-      __syncthreads();
+      atomicAdd(&cnt2[h >> 1], 1u << (16 * (h & 1)));  // counts stay below 2^16
+    }
+    if (flags && ok && pos + 1 < n) {
+      pairs++;
+      if (decode_index<PACKED>(r1, limit, j2, br, bp) && j2 == j) eq++;
     }
   }
+  if (flags) {
+    atomicAdd(&neq, eq);
+    atomicAdd(&npairs, pairs);
+  }
+  __syncthreads();
   // a bin is hot when it occurs at least twice in the sample
-  for (int k = tid; k < kSketch - 1; k += 1024) {
-    uint32_t v = s[k];
-    bool first = (k == 0 || s[k - 1] != v);
-    if (v != kHotEmpty && first && s[k + 1] == v) {
+  uint32_t dup = 0;
+  for (int h = tid; h < kSketchSlots; h += nthr) {
+    uint32_t c = (cnt2[h >> 1] >> (16 * (h & 1))) & 0xffffu;
+    if (c >= 2) {
+      dup += c;
       if (atomicAdd(&nhot, 1u) < (uint32_t)kHotMax) {
-        uint32_t h = hot_hash(v);
-        while (atomicCAS(&table[h], kHotEmpty, v) != kHotEmpty) h = (h + 1) & (kHotSlots - 1);
+        uint32_t v = keys[h], t = hot_hash(v);
+        while (atomicCAS(&table[t], kHotEmpty, v) != kHotEmpty) t = (t + 1) & (kHotSlots - 1);
       }
     }
   }
+  if (flags) atomicAdd(&ndup, dup);
   __syncthreads();
-  for (int k = tid; k < kHotSlots; k += 1024) hot_out[k] = table[k];
+  for (int k = tid; k < kHotSlots; k += nthr) hot_out[k] = table[k];
+  if (flags && tid == 0) {
+    // Worth the hot table when the samples that share their bin with another
+    // sample are at least 1/32 of the sample and 4x what uniform updates over
+    // `limit` bins would give by chance (k(k-1)/limit for a sample of k).
+    const double k = kSketch, chance = k * (k - 1.0) / (double)(limit ? limit : 1);
+    const bool skewed = ndup * 32u >= (uint32_t)kSketch && (double)ndup >= 4.0 * chance;
+    *flags = (skewed ? kFlagHotBins : 0u) | (npairs && neq * 8 >= npairs ? kFlagRunsSeen : 0u);
+  }
   __syncthreads();
 }
 
 template <bool PACKED>
 __global__ __launch_bounds__(1024) void k_hot_sketch(const int64_t* __restrict__ idx, int64_t n,
-                                                     uint64_t limit, uint32_t* __restrict__ hot_out) {
-  hot_sketch_body<PACKED>(idx, n, limit, hot_out);
+                                                     uint64_t limit, uint32_t* __restrict__ hot_out,
+                                                     uint32_t* __restrict__ flags) {
+  hot_sketch_body<PACKED>(idx, n, limit, hot_out, flags);
 }
 
 // Runs of equal keys in adjacent lanes of the wave.  Returns the lane that heads
@@ -165,10 +199,11 @@ __device__ inline void part_count_body(uint32_t* sh, int64_t c, const int64_t* _
                                        int64_t n, uint64_t limit, int shift, int B, int64_t chunk,
                                        uint32_t* __restrict__ rows,
                                        const uint32_t* __restrict__ hot_g, ErrWords* err) {
+  constexpr bool use_hot = HOT, use_runs = RUNS;
   uint32_t* hist = sh;              // [B]
   uint32_t* hot = sh + B;           // [kHotSlots] when HOT
   for (int b = threadIdx.x; b < B; b += blockDim.x) hist[b] = 0;
-  if (HOT)
+  if (use_hot)
     for (int k = threadIdx.x; k < kHotSlots; k += blockDim.x) hot[k] = hot_g[k];
   __syncthreads();
   int64_t lo = c * chunk;
@@ -190,8 +225,8 @@ __device__ inline void part_count_body(uint32_t* sh, int64_t c, const int64_t* _
       uint32_t key = kHotEmpty;
       uint64_t j;
       if (e < hi && decode_index<PACKED>(raw[u], limit, j, bad_r, bad_p))
-        if (!HOT || hot_find(hot, (uint32_t)j) < 0) key = (uint32_t)(j >> shift);
-      if (RUNS) {
+        if (!use_hot || hot_find(hot, (uint32_t)j) < 0) key = (uint32_t)(j >> shift);
+      if (use_runs) {
         // a run of >= kLongRun equal bins is added directly by the scatter
         // pass and is not partitioned (same aligned 64-element groups there)
         uint32_t bin = key != kHotEmpty ? (uint32_t)j : kHotEmpty, blen;
@@ -211,29 +246,54 @@ __device__ inline void part_count_body(uint32_t* sh, int64_t c, const int64_t* _
   flush_errors(err, bad_r, bad_p);
 }
 
+// `flags` (optional, written by the sketch) switches the HOT / RUNS handling
+// off when the sample saw no hot bins / no runs; the scatter pass reads the
+// same word, so both passes make the same choice.
 template <bool PACKED, bool HOT, bool RUNS>
 __global__ __launch_bounds__(1024) void k_part_count(const int64_t* __restrict__ idx, int64_t n,
                                                      uint64_t limit, int shift, int B,
                                                      int64_t chunk, uint32_t* __restrict__ rows,
                                                      const uint32_t* __restrict__ hot_g,
-                                                     ErrWords* err) {
+                                                     ErrWords* err,
+                                                     const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
-  part_count_body<PACKED, HOT, RUNS>(sh, blockIdx.x, idx, n, limit, shift, B, chunk, rows, hot_g,
-                                     err);
+  const int64_t c = blockIdx.x;
+  const uint32_t fl = flags ? *flags : (kFlagHotBins | kFlagRunsSeen);
+  const bool h = HOT && (fl & kFlagHotBins), r = RUNS && (fl & kFlagRunsSeen);
+  if (h && r)
+    part_count_body<PACKED, HOT, RUNS>(sh, c, idx, n, limit, shift, B, chunk, rows, hot_g, err);
+  else if (h)
+    part_count_body<PACKED, HOT, false>(sh, c, idx, n, limit, shift, B, chunk, rows, hot_g, err);
+  else if (r)
+    part_count_body<PACKED, false, RUNS>(sh, c, idx, n, limit, shift, B, chunk, rows, hot_g, err);
+  else
+    part_count_body<PACKED, false, false>(sh, c, idx, n, limit, shift, B, chunk, rows, hot_g, err);
 }
 
-// ---- pass 2a: column prefix over chunks (thread per bucket) ----------------------
-[[maybe_unused]] static __global__ __launch_bounds__(256) void k_part_offsets(uint32_t* __restrict__ rows, int C,
-                                                             int B, uint32_t* __restrict__ total) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  uint32_t run = 0;
-  for (int c = 0; c < C; c++) {
+// ---- pass 2a: column prefix over chunks (a workgroup of 256 per bucket) -----------
+// rows[c][b] becomes the items of bucket b in chunks before c; total[b] the sum.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_part_offsets(
+    uint32_t* __restrict__ rows, int C, int B, uint32_t* __restrict__ total) {
+  __shared__ uint32_t part[256];
+  const int b = blockIdx.x, t = threadIdx.x, T = blockDim.x;
+  const int per = (C + T - 1) / T, c0 = t * per;
+  uint32_t sum = 0;
+  for (int c = c0; c < c0 + per && c < C; c++) sum += rows[(size_t)c * B + b];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    uint32_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = t ? part[t - 1] : 0;
+  for (int c = c0; c < c0 + per && c < C; c++) {
     uint32_t v = rows[(size_t)c * B + b];
     rows[(size_t)c * B + b] = run;
     run += v;
   }
-  total[b] = run;
+  if (t == T - 1) total[b] = part[T - 1];
 }
 
 // ---- pass 2b: prefix over buckets (one workgroup, any size up to 1024) ------------
@@ -302,7 +362,7 @@ inline int part_pick_depth(int B, size_t esize, size_t budget, bool hot) {
   return part_scatter_lds(B, S, esize, hot) <= budget ? S : 0;
 }
 
-constexpr int kScatterU = 4;  // elements per thread per round
+constexpr int kScatterU = 8;  // elements per thread per round (loaded a round ahead)
 
 // MakeEntry(e, j, low) -> entry stored for element e with decoded index j
 // (low = j & ((1<<shift)-1)).  HotFlush(bin, count) adds count to a bin
@@ -320,6 +380,7 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
                                          const uint32_t* __restrict__ start,
                                          const uint32_t* __restrict__ hot_g, Store store,
                                          MakeEntry make, HotFlush hot_flush, Skip skip) {
+  constexpr bool use_hot = HOT, use_runs = RUNS;
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* cursor = (uint32_t*)smem;  // [B] next staging slot of this chunk per bucket
   uint32_t* fill = cursor + B;         // [B]
@@ -336,7 +397,7 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
     cursor[b] = start[b] + row[b];
     fill[b] = 0;
   }
-  if (HOT)
+  if (use_hot)
     for (int k = tid; k < kHotSlots; k += nthr) {
       hkeys[k] = hot_g[k];
       hcnt[k] = 0;
@@ -418,11 +479,11 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
       if (e < hi) {
         unsigned br = 0, bp = 0;
         ok = decode_index<PACKED>(raw[u], limit, j, br, bp);
-        if (ok && HOT) hs = hot_find(hkeys, (uint32_t)j);
+        if (ok && use_hot) hs = hot_find(hkeys, (uint32_t)j);
         if (!ok) skip(e);
       }
-      if (HOT) {
-        if (RUNS) {
+      if (use_hot) {
+        if (use_runs) {
           uint32_t len;
           int head = wave_runs(hs >= 0 ? (uint32_t)hs : kHotEmpty, len);
           if (hs >= 0 && lane == head) atomicAdd(&hcnt[hs], len);
@@ -431,7 +492,7 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
         }
       }
       bool cold = ok && hs < 0;
-      if (RUNS) {
+      if (use_runs) {
         // long runs of one bin are added directly (the count pass skips them)
         uint32_t len;
         int head = wave_runs(cold ? (uint32_t)j : kHotEmpty, len);
@@ -453,7 +514,7 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
 #pragma unroll
       for (int u = 0; u < kScatterU; u++) {
         uint32_t slot = 0;
-        if (RUNS) {
+        if (use_runs) {
           uint32_t len;
           int head = wave_runs(pending[u] ? (uint32_t)bk[u] : kHotEmpty, len);
           uint32_t first = 0;
@@ -488,9 +549,38 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
     uint32_t f = fill[b];
     if (f) ship(b, f);
   }
-  if (HOT)
+  if (use_hot)
     for (int k = tid; k < kHotSlots; k += nthr)
       if (hcnt[k]) hot_flush(hkeys[k], hcnt[k]);
+}
+
+// The scatter pass under the count pass's choice from `flags` (see
+// k_part_count).  Without the hot table, its LDS goes to deeper buffers:
+// S_cold entries per bucket (the caller sized the allocation for both).
+template <bool PACKED, bool HOT, bool RUNS, typename E, typename MakeEntry, typename HotFlush,
+          typename Store, typename Skip>
+__device__ inline void part_scatter_adaptive(int64_t c, const int64_t* __restrict__ idx, int64_t n,
+                                             uint64_t limit, int shift, int B, int S, int S_cold,
+                                             int64_t chunk, const uint32_t* __restrict__ rows,
+                                             const uint32_t* __restrict__ start,
+                                             const uint32_t* __restrict__ hot_g, Store store,
+                                             MakeEntry make, HotFlush hot_flush, Skip skip,
+                                             const uint32_t* __restrict__ flags) {
+  const uint32_t fl = flags ? *flags : (kFlagHotBins | kFlagRunsSeen);
+  const bool h = HOT && (fl & kFlagHotBins), r = RUNS && (fl & kFlagRunsSeen);
+  const int Sc = S_cold > 0 ? S_cold : S;
+  if (h && r)
+    part_scatter_body<PACKED, HOT, RUNS, E>(c, idx, n, limit, shift, B, S, chunk, rows, start,
+                                            hot_g, store, make, hot_flush, skip);
+  else if (h)
+    part_scatter_body<PACKED, HOT, false, E>(c, idx, n, limit, shift, B, S, chunk, rows, start,
+                                             hot_g, store, make, hot_flush, skip);
+  else if (r)
+    part_scatter_body<PACKED, false, RUNS, E>(c, idx, n, limit, shift, B, HOT ? Sc : S, chunk,
+                                              rows, start, hot_g, store, make, hot_flush, skip);
+  else
+    part_scatter_body<PACKED, false, false, E>(c, idx, n, limit, shift, B, HOT ? Sc : S, chunk,
+                                               rows, start, hot_g, store, make, hot_flush, skip);
 }
 
 }  // namespace bale

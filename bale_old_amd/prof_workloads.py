@@ -1,20 +1,23 @@
 """One dispatch of each measured kernel, for hardware-counter runs:
 
-    rocprofv3 --pmc FETCH_SIZE -- python3 -m bale_old_amd.prof_workloads gather copy histo
+    rocprofv3 --pmc FETCH_SIZE -- python3 bale_old_amd/prof_workloads.py gather copy histo
 
   gather  k_gather_direct over 2^28 uniform requests into a 2^28-entry table
   copy    k_probe_copy of exactly 1 GiB (read 1 GiB, write 1 GiB): the
           calibration that converts the counters to bytes
-  histo   the bucketed histogram of 2^28 uniform updates into 2^26 int64 bins
-  zipf    the bucketed histogram with hot bins on Zipf(1.1) updates
+  histo   the planned (bucketed) histogram of 2^28 uniform updates into 2^26 int64 bins
+  zipf    the same plan on 2^28 Zipf(1.1) updates (its sample switches the hot table on)
 """
 from __future__ import annotations
 
 import sys
+from pathlib import Path
 
 import torch
 
-import bale_old_amd as B
+if __package__ in (None, ""):  # run by file path: make the package importable
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+import bale_old_amd as B  # noqa: E402
 
 
 def main(argv=None) -> int:
@@ -43,7 +46,7 @@ def main(argv=None) -> int:
     if "zipf" in which:
         counts = torch.zeros(1 << 26, dtype=torch.int64, device=dev)
         B.gen_index(idx, 1 << 26, dist=1, s=1.1, seed=3)
-        B.histo(idx, counts, cfg=B.config(B.BUCKET, block=1024, hot=1))
+        B.histo(idx, counts, cfg=B.histo_plan(n, 1 << 26))
         del counts
     torch.cuda.synchronize()
     print("prof_workloads done:", " ".join(sorted(which)), flush=True)

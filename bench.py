@@ -256,13 +256,13 @@ def cpu_baseline(idx_t, table_t, max_threads=16):
 
 def load_traffic(name: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    p = ROOT / "profiles" / f"{name}_traffic.json"
+    p = ROOT / "profiles" / f"{name}_traffic.txt"
     if not p.exists():
         return None
-    try:
-        return json.loads(p.read_text()).get("traffic_bytes")
-    except Exception:
-        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes:"):
+            return int(line.split(":", 1)[1])
+    return None
 
 
 def main() -> int:

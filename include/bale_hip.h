@@ -107,7 +107,9 @@ typedef struct bale_config {
   int32_t merge;      /* BALE_MERGE_*; -1 = default */
   int32_t groups_per_cu; /* resident workgroups per CU for persistent passes; 0 = default */
   int32_t hot;        /* histogram DIRECT/BUCKET: 1 = count the bins a 4096-index sample
-                         sees twice in a per-workgroup LDS hash table; 0 = off */
+                         sees twice in a per-workgroup LDS hash table; 0 = off;
+                         BUCKET only: 2 = let that sample decide on the device, per
+                         call, whether to use the hot table and run merging */
   int32_t task_log2;  /* batches: operations per task (log2, 12..22); 0 = 16 */
 } bale_config;
 

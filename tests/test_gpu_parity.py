@@ -68,7 +68,8 @@ HISTO_CASES = [
 ]
 STRATS = [(B.DIRECT, 0, 0), (B.DIRECT, 1, 0), (B.DIRECT, 2, 0), (B.DIRECT, 0, 1),
           (B.LDS, 0, 0), (B.LDS, 1, 0), (B.LDS, 2, 0),
-          (B.BUCKET, 0, 0), (B.BUCKET, 0, 1), (B.AUTO, 0, 0)]
+          (B.BUCKET, 0, 0), (B.BUCKET, 0, 1), (B.BUCKET, 1, 1), (B.BUCKET, 0, 2),
+          (B.AUTO, 0, 0)]
 
 
 @pytest.mark.parametrize("case", HISTO_CASES, ids=lambda c: f"b{c[0]}_n{c[1]}_d{c[2]}_r{c[4]}")
@@ -95,7 +96,7 @@ def test_histo_tiles(dev, orc, tile_log2):
     nbins, n = 100_003, 1 << 19
     idx = gen(dev, n, nbins, 1, 1.05, 8, seed=tile_log2)
     want = np.bincount(to_np(idx), minlength=nbins)
-    for strat, hot in ((B.LDS, 0), (B.BUCKET, 0), (B.BUCKET, 1)):
+    for strat, hot in ((B.LDS, 0), (B.BUCKET, 0), (B.BUCKET, 1), (B.BUCKET, 2)):
         counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
         B.histo(idx, counts, cfg=B.config(strat, block=1024, tile_log2=tile_log2, hot=hot))
         assert np.array_equal(to_np(counts), want), (strat, hot)
@@ -118,10 +119,10 @@ def test_histo_packed_and_errors(dev, orc):
     nbad = 2 * len(range(0, n, 1000))
     good = to_np(bad)
     good = good[(good >= 0) & (good < nbins)]
-    for strat in (B.DIRECT, B.LDS, B.BUCKET):
+    for strat, hot in ((B.DIRECT, 0), (B.LDS, 0), (B.BUCKET, 0), (B.BUCKET, 2)):
         counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
         with pytest.raises(B.BaleError) as ei:
-            B.histo(bad, counts, cfg=B.config(strat, block=1024))
+            B.histo(bad, counts, cfg=B.config(strat, block=1024, hot=hot))
         assert ei.value.code == B.ERR_RANGE
         assert str(nbad) in str(ei.value)
         assert np.array_equal(to_np(counts), np.bincount(good, minlength=nbins))
