@@ -114,11 +114,21 @@ __device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t 
   if (tid == 0) nhot = neq = npairs = ndup = 0;
   __syncthreads();
   uint32_t eq = 0, pairs = 0;
-  for (int k = tid; k < kSketch; k += nthr) {
-    int64_t pos = (int64_t)(((unsigned __int128)k * (uint64_t)n) / kSketch);
-    // both loads issued before either is used
-    const int64_t r0 = pos < n ? idx[pos] : 0;
-    const int64_t r1 = flags && pos + 1 < n ? idx[pos + 1] : 0;
+  // every sample (and its right neighbour) is loaded before any is used
+  constexpr int kPer = kSketch / 1024;  // launched with 1024 threads
+  int64_t r0s[kPer], r1s[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int k = tid + q * 1024;
+    const int64_t pos = (int64_t)(((unsigned __int128)k * (uint64_t)n) / kSketch);
+    r0s[q] = pos < n ? idx[pos] : 0;
+    r1s[q] = flags && pos + 1 < n ? idx[pos + 1] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int k = tid + q * 1024;
+    const int64_t pos = (int64_t)(((unsigned __int128)k * (uint64_t)n) / kSketch);
+    const int64_t r0 = r0s[q], r1 = r1s[q];
     uint64_t j = 0, j2 = 0;
     unsigned br = 0, bp = 0;
     bool ok = pos < n && decode_index<PACKED>(r0, limit, j, br, bp);
