@@ -431,7 +431,7 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
         b_l = list[k];
         pos_l = cursor[b_l];
         cursor[b_l] = pos_l + (uint32_t)S;
-        fill[b_l] = 0;
+        fill[b_l] -= (uint32_t)S;  // entries past S (overflow) stay counted
       }
       const uint32_t cnt = min(64u, (nl - g0 + nwaves - 1) / (uint32_t)nwaves);
       if (S <= 64) {
@@ -517,41 +517,56 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
         pending[u] = true;
       }
     }
+    // Place: one slot per element.  A slot past the buffer (overflow) is
+    // kept and used after the buffer is shipped.
+    uint32_t slot[kScatterU];
     bool any = false;
 #pragma unroll
-    for (int u = 0; u < kScatterU; u++) any = any || pending[u];
-    while (__syncthreads_or(any)) {
-#pragma unroll
-      for (int u = 0; u < kScatterU; u++) {
-        uint32_t slot = 0;
-        if (use_runs) {
-          uint32_t len;
-          int head = wave_runs(pending[u] ? (uint32_t)bk[u] : kHotEmpty, len);
-          uint32_t first = 0;
-          if (pending[u] && lane == head) first = atomicAdd(&fill[bk[u]], len);
-          first = __shfl(first, head);
-          slot = first + (uint32_t)(lane - head);
-        } else if (pending[u]) {
-          slot = atomicAdd(&fill[bk[u]], 1u);
-        }
-        if (pending[u]) {
-          if (slot < (uint32_t)S) {
-            buf[(size_t)bk[u] * S + slot] = ent[u];
-            pending[u] = false;
-            if (slot == (uint32_t)S - 1) list[atomicAdd(&nlist[p], 1u)] = (uint32_t)bk[u];
-          }
-        }
+    for (int u = 0; u < kScatterU; u++) {
+      slot[u] = 0;
+      if (use_runs) {
+        uint32_t len;
+        int head = wave_runs(pending[u] ? (uint32_t)bk[u] : kHotEmpty, len);
+        uint32_t first = 0;
+        if (pending[u] && lane == head) first = atomicAdd(&fill[bk[u]], len);
+        first = __shfl(first, head);
+        slot[u] = first + (uint32_t)(lane - head);
+      } else if (pending[u]) {
+        slot[u] = atomicAdd(&fill[bk[u]], 1u);
       }
+      if (pending[u] && slot[u] < (uint32_t)S) {
+        buf[(size_t)bk[u] * S + slot[u]] = ent[u];
+        pending[u] = false;
+        if (slot[u] == (uint32_t)S - 1) list[atomicAdd(&nlist[p], 1u)] = (uint32_t)bk[u];
+      }
+      any = any || pending[u];
+    }
+    for (;;) {
       __syncthreads();
-      // ships reset fill[] of what they ship; the other list length (last used
-      // a step ago, whose readers are past the barrier above) is cleared here
+      // ships take S entries off each listed bucket; the other list length
+      // (last used a round ago, whose readers are past the barrier above) is
+      // cleared here
       ship_listed(nlist[p]);
       if (tid == 0) nlist[p ^ 1] = 0;
       p ^= 1;
+      // ships have read their buffers before overflow entries (and the next
+      // round's placements) are written into them
+      if (!__syncthreads_or(any)) break;
       any = false;
 #pragma unroll
-      for (int u = 0; u < kScatterU; u++) any = any || pending[u];
-      // the next __syncthreads_or orders the ships and resets before any new append
+      for (int u = 0; u < kScatterU; u++) {
+        if (pending[u]) {
+          slot[u] -= (uint32_t)S;
+          if (slot[u] < (uint32_t)S) {
+            buf[(size_t)bk[u] * S + slot[u]] = ent[u];
+            pending[u] = false;
+            if (slot[u] == (uint32_t)S - 1) list[atomicAdd(&nlist[p], 1u)] = (uint32_t)bk[u];
+          }
+        }
+        any = any || pending[u];
+      }
+      // more than S entries past a full buffer: ship again (rare)
+      if (!__syncthreads_or(any)) break;
     }
   }
   __syncthreads();
