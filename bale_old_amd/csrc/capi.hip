@@ -110,7 +110,9 @@ static void plan_histo(const Device* d, int64_t n, int64_t nbins, int dtype, bal
     c->strategy = BALE_STRAT_BUCKET;
     c->block = 1024;
     c->hot = 2;
-    c->tile_log2 = nbins <= ((int64_t)1 << 22) ? 14 : 0;
+    // bucket width: narrower buckets flush cheaper on small tables, wider ones
+    // (2^16, packed 16-bit apply tile) halve the scatter's buckets on big ones
+    c->tile_log2 = nbins <= ((int64_t)1 << 22) ? 14 : nbins > ((int64_t)1 << 25) ? 16 : 0;
   } else {
     c->strategy = BALE_STRAT_DIRECT;
     c->block = 256;

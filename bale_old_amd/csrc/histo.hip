@@ -210,10 +210,66 @@ __device__ inline void lds_add8(uint32_t* sh, u32x4 x) {
   atomicAdd(&sh[cur], cnt);
 }
 
+// Packed tile for 2^16-bin buckets: two 16-bit counters per LDS word (bin k in
+// half k & 1 of word k >> 1), 128 KiB for the whole bucket.  A counter never
+// wraps: an add that carries its counter across a multiple of kSpill moves
+// kSpill to the device counter and takes it off the LDS word.  With U the
+// counter's value less kSpill per correction still owed, every add keeps
+// U < kSpill, so a counter reaches 2^16 only if 64 of its corrections are
+// owed at once, i.e. about 2^16 - kSpill adds landed on that one bin (at most
+// 8 per LDS atomic, one LDS atomic per bank per cycle) between one lane's
+// atomic returning and that lane's correction: thousands of cycles of a
+// wave that is ready to issue.
+constexpr uint32_t kSpill = 1024;
+
+template <typename CT>
+__device__ inline void pk_fix(uint32_t* sh, uint32_t bin, uint32_t inc, uint32_t old, CT* dst,
+                              uint32_t* spilled) {
+  const uint32_t sft = (bin & 1u) * 16u;
+  if ((((old >> sft) & 0xffffu) & (kSpill - 1u)) + inc >= kSpill) {
+    atomicAdd(&sh[bin >> 1], 0u - (kSpill << sft));
+    atomic_add_dev(dst + bin, (CT)kSpill);
+    *spilled = 1u;
+  }
+}
+
+// lds_add8 for the packed tile; all of a call's atomics are issued before
+// their results are checked.
+template <bool RUNS, typename CT>
+__device__ inline void lds_add8_pk(uint32_t* sh, u32x4 x, CT* dst, uint32_t* spilled) {
+  uint32_t v[8] = {x.x & 0xffff, x.x >> 16, x.y & 0xffff, x.y >> 16,
+                   x.z & 0xffff, x.z >> 16, x.w & 0xffff, x.w >> 16};
+  uint32_t inc[8], old[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) inc[i] = 1;
+  if (RUNS) {
+    // a run's count goes on its first entry; the others add nothing
+#pragma unroll
+    for (int i = 6; i >= 0; i--)
+      if (v[i] == v[i + 1]) {
+        inc[i] += inc[i + 1];
+        inc[i + 1] = 0;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    old[i] = inc[i] ? atomicAdd(&sh[v[i] >> 1], inc[i] << ((v[i] & 1u) * 16u)) : 0u;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    if (inc[i]) pk_fix(sh, v[i], inc[i], old[i], dst, spilled);
+}
+
+template <typename CT>
+__device__ inline void lds_add1_pk(uint32_t* sh, uint32_t bin, CT* dst, uint32_t* spilled) {
+  const uint32_t old = atomicAdd(&sh[bin >> 1], 1u << ((bin & 1u) * 16u));
+  pk_fix(sh, bin, 1u, old, dst, spilled);
+}
+
 // Persistent: each workgroup takes segments (<= seg_len entries of one bucket)
 // from a queue, histograms them in LDS and adds the tile into counts.  A bucket
 // with one segment is owned by one workgroup, so its flush is a plain add.
-template <typename CT, bool RUNS>
+// PK: the bucket's tile is packed 16-bit counters (2^16-bin buckets).
+template <typename CT, bool RUNS, bool PK>
 __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict__ staged,
                                                       const uint32_t* __restrict__ start,
                                                       const uint32_t* __restrict__ segstart,
@@ -223,6 +279,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
                                                       const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
   __shared__ uint32_t s_work[4];
+  __shared__ uint32_t s_spilled;  // PK: some counter of this segment spilled
   const bool merge = RUNS && (!flags || (*flags & kFlagRunsSeen));
   const int tid = threadIdx.x, nthr = blockDim.x;
   const uint32_t total = misc[0];
@@ -241,6 +298,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
       }
       s_work[0] = w;
       s_work[1] = b;
+      s_spilled = 0;
     }
     __syncthreads();
     const uint32_t w = s_work[0], b = s_work[1];
@@ -251,30 +309,41 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
     const uint32_t e1 = min(start[b + 1], e0 + seg_len);
     const uint64_t lo_bin = (uint64_t)b << shift;
     const uint32_t width = (uint32_t)min((uint64_t)1 << shift, nbins - lo_bin);
-    for (uint32_t k = tid; k < width; k += nthr) sh[k] = 0;
+    const uint32_t words = PK ? (width + 1u) >> 1 : width;
+    CT* dst = counts + lo_bin;
+    for (uint32_t k = tid; k < words; k += nthr) sh[k] = 0;
     __syncthreads();
     // unaligned head, then 8 entries per 16-byte load
     uint32_t a0 = min(e1, (e0 + 7u) & ~7u);
-    for (uint32_t k = e0 + tid; k < a0; k += nthr) atomicAdd(&sh[staged[k]], 1u);
+    for (uint32_t k = e0 + tid; k < a0; k += nthr) {
+      if (PK) lds_add1_pk(sh, staged[k], dst, &s_spilled);
+      else atomicAdd(&sh[staged[k]], 1u);
+    }
     uint32_t nvec = (e1 - a0) / 8u;
     const u32x4* v = (const u32x4*)(staged + a0);
     for (uint32_t q = tid; q < nvec; q += nthr) {
       u32x4 x = __builtin_nontemporal_load(v + q);
-      if (merge) lds_add8<true>(sh, x);
-      else lds_add8<false>(sh, x);
-    }
-    for (uint32_t k = a0 + nvec * 8u + tid; k < e1; k += nthr) atomicAdd(&sh[staged[k]], 1u);
-    __syncthreads();
-    CT* dst = counts + lo_bin;
-    if (nseg == 1) {
-      for (uint32_t k = tid; k < width; k += nthr) {
-        uint32_t c = sh[k];
-        if (c) dst[k] += (CT)c;
+      if (PK) {
+        if (merge) lds_add8_pk<true>(sh, x, dst, &s_spilled);
+        else lds_add8_pk<false>(sh, x, dst, &s_spilled);
+      } else {
+        if (merge) lds_add8<true>(sh, x);
+        else lds_add8<false>(sh, x);
       }
-    } else {
-      for (uint32_t k = tid; k < width; k += nthr) {
-        uint32_t c = sh[k];
-        if (c) atomic_add_dev(dst + k, (CT)c);
+    }
+    for (uint32_t k = a0 + nvec * 8u + tid; k < e1; k += nthr) {
+      if (PK) lds_add1_pk(sh, staged[k], dst, &s_spilled);
+      else atomicAdd(&sh[staged[k]], 1u);
+    }
+    __syncthreads();
+    // a bucket with one segment is this workgroup's alone: plain adds, unless
+    // a spill already went to its counters atomically
+    const bool plain = nseg == 1 && !(PK && s_spilled);
+    for (uint32_t k = tid; k < width; k += nthr) {
+      uint32_t c = PK ? (sh[k >> 1] >> ((k & 1u) * 16u)) & 0xffffu : sh[k];
+      if (c) {
+        if (plain) dst[k] += (CT)c;
+        else atomic_add_dev(dst + k, (CT)c);
       }
     }
     __syncthreads();
@@ -407,14 +476,19 @@ template <bool PACKED, bool HOT, bool RUNS, typename CT>
 static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins,
                         const bale_config& c, Device* d, hipStream_t s) {
   int tb = c.tile_log2 > 0 ? c.tile_log2 : max_tile_bits(d);
-  if (tb > max_tile_bits(d)) tb = max_tile_bits(d);
-  if (tb > 16) tb = 16;  // entries are 16-bit bin offsets
+  // entries are 16-bit bin offsets; a bucket one bit wider than a 32-bit
+  // tile allows is applied through the packed 16-bit tile
+  const int tb_max = std::min(16, max_tile_bits(d) + 1);
+  if (tb > tb_max) tb = tb_max;
+  const bool pk = tb > max_tile_bits(d);
   int64_t B = ceil_div(nbins, (int64_t)1 << tb);
   if (B > kMaxBuckets) return (int)hipErrorInvalidValue;
-  const size_t budget = (size_t)d->lds_per_block;
+  // groups_per_cu scatter workgroups share a CU's LDS, so one waits on
+  // memory while another places
+  int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
+  const size_t budget = (size_t)d->lds_per_block / (size_t)gpc;
   int S = part_pick_depth((int)B, sizeof(uint16_t), budget, HOT);
   if (S == 0) return (int)hipErrorInvalidValue;
-  int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
   int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
   // chunks are whole multiples of kChunkAlign so both passes see the same
   // aligned 64-update groups
@@ -453,12 +527,19 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   hipLaunchKernelGGL((k_histo_scatter<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024), lds, s, idx,
                      n, (uint64_t)nbins, tb, (int)B, S, chunk, L.chunk, L.start, L.hot,
                      (uint16_t*)L.staged, counts, (const uint32_t*)flags, S_cold);
-  size_t lds_apply = (size_t)4 << tb;
+  size_t lds_apply = (size_t)(pk ? 2 : 4) << tb;
   int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
-  allow_lds(k_histo_apply<CT, RUNS>, lds_apply);
-  hipLaunchKernelGGL((k_histo_apply<CT, RUNS>), dim3((unsigned)groups), dim3(1024), lds_apply, s,
-                     (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B, (uint32_t)seg,
-                     tb, (uint64_t)nbins, counts, (const uint32_t*)flags);
+  if (pk) {
+    allow_lds(k_histo_apply<CT, RUNS, true>, lds_apply);
+    hipLaunchKernelGGL((k_histo_apply<CT, RUNS, true>), dim3((unsigned)groups), dim3(1024),
+                       lds_apply, s, (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B,
+                       (uint32_t)seg, tb, (uint64_t)nbins, counts, (const uint32_t*)flags);
+  } else {
+    allow_lds(k_histo_apply<CT, RUNS, false>, lds_apply);
+    hipLaunchKernelGGL((k_histo_apply<CT, RUNS, false>), dim3((unsigned)groups), dim3(1024),
+                       lds_apply, s, (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B,
+                       (uint32_t)seg, tb, (uint64_t)nbins, counts, (const uint32_t*)flags);
+  }
   return (int)hipGetLastError();
 }
 

@@ -362,7 +362,7 @@ __device__ inline void part_scan_body(const uint32_t* __restrict__ count, int B,
 // ---- pass 3: scatter through LDS send buffers -------------------------------------
 // LDS layout: cursor[B], fill[B], list[B], nlist[2], [hot keys, hot counts], buf[B*S].
 inline size_t part_scatter_lds(int B, int S, size_t esize, bool hot) {
-  size_t head = (size_t)(3 * B + 2) * 4 + (hot ? (size_t)2 * kHotSlots * 4 : 0);
+  size_t head = (size_t)(3 * B + 2 + 64) * 4 + (hot ? (size_t)2 * kHotSlots * 4 : 0);
   return ((head + 15) & ~(size_t)15) + (size_t)B * S * esize;
 }
 // Largest power-of-two buffer depth S in [4, 256] that fits `budget` bytes (0 if none).
@@ -396,9 +396,10 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
   uint32_t* fill = cursor + B;         // [B]
   uint32_t* list = fill + B;           // [B] buckets to ship this step
   uint32_t* nlist = list + B;          // [2] list lengths, alternating between steps
-  uint32_t* hkeys = nlist + 2;         // [kHotSlots]
+  uint32_t* sink = nlist + 2;          // [64] per-lane target of placements that place nothing
+  uint32_t* hkeys = sink + 64;         // [kHotSlots]
   uint32_t* hcnt = hkeys + kHotSlots;  // [kHotSlots]
-  size_t head = (size_t)(3 * B + 2) * 4 + (HOT ? (size_t)2 * kHotSlots * 4 : 0);
+  size_t head = (size_t)(3 * B + 2 + 64) * 4 + (HOT ? (size_t)2 * kHotSlots * 4 : 0);
   E* buf = (E*)(smem + ((head + 15) & ~(size_t)15));  // [B*S]
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = __lane_id(), wave = tid >> 6, nwaves = nthr >> 6;
@@ -518,22 +519,23 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
       }
     }
     // Place: one slot per element.  A slot past the buffer (overflow) is
-    // kept and used after the buffer is shipped.
+    // kept and used after the buffer is shipped.  All of a round's slot
+    // atomics are issued before any result is used (lanes with nothing to
+    // place add to their own sink word, so no lane branches around them).
     uint32_t slot[kScatterU];
+    int rhead[kScatterU];
+#pragma unroll
+    for (int u = 0; u < kScatterU; u++) {
+      uint32_t len = 1;
+      rhead[u] = lane;
+      if (use_runs) rhead[u] = wave_runs(pending[u] ? (uint32_t)bk[u] : kHotEmpty, len);
+      const bool adds = pending[u] && lane == rhead[u];
+      slot[u] = atomicAdd(adds ? &fill[bk[u]] : &sink[lane], len);
+    }
     bool any = false;
 #pragma unroll
     for (int u = 0; u < kScatterU; u++) {
-      slot[u] = 0;
-      if (use_runs) {
-        uint32_t len;
-        int head = wave_runs(pending[u] ? (uint32_t)bk[u] : kHotEmpty, len);
-        uint32_t first = 0;
-        if (pending[u] && lane == head) first = atomicAdd(&fill[bk[u]], len);
-        first = __shfl(first, head);
-        slot[u] = first + (uint32_t)(lane - head);
-      } else if (pending[u]) {
-        slot[u] = atomicAdd(&fill[bk[u]], 1u);
-      }
+      if (use_runs) slot[u] = __shfl(slot[u], rhead[u]) + (uint32_t)(lane - rhead[u]);
       if (pending[u] && slot[u] < (uint32_t)S) {
         buf[(size_t)bk[u] * S + slot[u]] = ent[u];
         pending[u] = false;

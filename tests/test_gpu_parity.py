@@ -91,7 +91,7 @@ def test_histo_strategies_match_oracle(dev, orc, case, dtype):
             assert np.array_equal(to_np(counts), want), (strat, merge, hot, unroll, block)
 
 
-@pytest.mark.parametrize("tile_log2", [9, 12, 15])
+@pytest.mark.parametrize("tile_log2", [9, 12, 15, 16])
 def test_histo_tiles(dev, orc, tile_log2):
     nbins, n = 100_003, 1 << 19
     idx = gen(dev, n, nbins, 1, 1.05, 8, seed=tile_log2)
@@ -100,6 +100,30 @@ def test_histo_tiles(dev, orc, tile_log2):
         counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
         B.histo(idx, counts, cfg=B.config(strat, block=1024, tile_log2=tile_log2, hot=hot))
         assert np.array_equal(to_np(counts), want), (strat, hot)
+
+
+@pytest.mark.parametrize("case", ["one_bin", "zipf1.4", "runs64"])
+@pytest.mark.parametrize("dtype", [torch.int64, torch.int32])
+def test_histo_packed_tile_spills(dev, case, dtype):
+    """2^16-bin buckets use 16-bit LDS counters that spill every 1024 adds:
+    drive single bins far past 2^16 adds within one bucket segment."""
+    if case == "one_bin":
+        nbins, n = (1 << 17) + 3, 1 << 21
+        idx = torch.full((n,), 70001, dtype=torch.int64, device=dev)
+        idx[::3] = 131074  # second bucket, last (partial) tile
+    elif case == "zipf1.4":
+        nbins, n = (1 << 18) + 5, 1 << 21
+        idx = gen(dev, n, nbins, 1, 1.4, 0, seed=17)
+    else:
+        nbins, n = (1 << 18) + 5, 1 << 21
+        idx = gen(dev, n, nbins, 1, 1.2, 64, seed=18)
+    want = np.bincount(to_np(idx), minlength=nbins)
+    assert want.max() > 1 << 16
+    for merge, hot in ((0, 0), (1, 0), (0, 2)):
+        counts = torch.full((nbins,), 5, dtype=dtype, device=dev)
+        cfg = B.config(B.BUCKET, block=1024, tile_log2=16, merge=merge, hot=hot)
+        assert B.histo(idx, counts, cfg=cfg) == 0
+        assert np.array_equal(to_np(counts).astype(np.int64), want + 5), (case, merge, hot)
 
 
 def test_histo_packed_and_errors(dev, orc):
