@@ -115,6 +115,15 @@ def _proto(L) -> None:
     f("bale_all_free", None, VP)
     f("bale_memput", C.c_int, VP, VP, I64T)
     f("bale_memget", C.c_int, VP, VP, I64T)
+    f("bale_convey_new", VP, I64T, I64T, C.c_size_t)
+    f("bale_convey_begin", C.c_int, VP, C.c_size_t)
+    f("bale_convey_push", C.c_int, VP, VP, VP, I64T, VP)
+    f("bale_convey_advance", C.c_int, VP, C.c_int, VP)
+    f("bale_convey_pull", C.c_int, VP, I64T, C.POINTER(VP), P64, P64)
+    f("bale_convey_unpull", C.c_int, VP, I64T, I64T)
+    f("bale_convey_reset", C.c_int, VP)
+    f("bale_convey_free", C.c_int, VP)
+    f("bale_memcpy_d2d", C.c_int, VP, VP, I64T, VP)
     f("histo_agi", C.c_double, VP, I64T, VP)
     f("histo_conveyor", C.c_double, VP, I64T, VP)
     for name in DROPIN_HISTO_BUF:

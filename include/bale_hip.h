@@ -23,6 +23,7 @@
  */
 #ifndef BALE_HIP_H
 #define BALE_HIP_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -213,6 +214,50 @@ int bale_probe_random_read(const int64_t *table, int64_t n, int64_t nreads,
  * nt, bit 4 sc1; one of 0,1,2,3,16,17,18,19); table at most 4 GiB. */
 int bale_probe_random_read_policy(const int64_t *table, int64_t n, int64_t nreads,
                                   int64_t *out, int policy, void *stream);
+
+/* ---- device conveyor ---------------------------------------------------
+ * The conveyor interface (convey/convey.h:184-338) as a bulk service on one
+ * GPU: items go to up to 256 virtual PEs, and each PE receives its items in
+ * push order (the one-source FIFO guarantee, convey/narrative.h:135-137).
+ * Items and PE numbers are device memory; a delivery happens in
+ * bale_convey_advance, after which each PE's items are one contiguous device
+ * span.  Return codes follow convey.h:48-53: */
+#define BALE_CONVEY_DONE 0 /* advance: done, everything pulled */
+#define BALE_CONVEY_FAIL 0 /* push: full until the next delivery; pull: nothing */
+#define BALE_CONVEY_OK 1
+/* negative: a BALE_ERR_* code (bale_hip_last_error() says why). */
+typedef struct bale_convey bale_convey;
+/* convey.h:184 convey_new -- n_pe virtual PEs (1..256), room for `capacity`
+ * pushed items between deliveries, items of at most max_item_bytes (4, 8,
+ * 12, 16, 24 or 32).  NULL on error. */
+bale_convey *bale_convey_new(int64_t n_pe, int64_t capacity, size_t max_item_bytes);
+/* convey.h:225 convey_begin -- dormant -> working, with this item size. */
+int bale_convey_begin(bale_convey *c, size_t item_bytes);
+/* convey.h:237 convey_push, in bulk -- copies n items and their PE numbers
+ * (device arrays) into the conveyor on `stream`.  BALE_CONVEY_FAIL if they do
+ * not fit before the next delivery (nothing is copied then). */
+int bale_convey_push(bale_convey *c, const void *items, const int64_t *pes, int64_t n,
+                     void *stream);
+/* convey.h:318 convey_advance -- delivers everything pushed once the previous
+ * delivery has been pulled (waits for the GPU).  Returns BALE_CONVEY_DONE when
+ * `done` is set and nothing is left to push or pull, else BALE_CONVEY_OK;
+ * BALE_ERR_RANGE when items named a PE outside 0..n_pe-1 (they are dropped,
+ * the rest are delivered). */
+int bale_convey_advance(bale_convey *c, int done, void *stream);
+/* convey.h:278 convey_apull, in bulk -- *items = device pointer to the items
+ * delivered to `pe` and not pulled yet, *count of them, *from = 0 (the only
+ * source).  BALE_CONVEY_OK if count > 0, BALE_CONVEY_FAIL if none. */
+int bale_convey_pull(bale_convey *c, int64_t pe, const void **items, int64_t *count,
+                     int64_t *from);
+/* convey.h:301 convey_unpull -- gives back the last `count` items of the
+ * latest pull from `pe`. */
+int bale_convey_unpull(bale_convey *c, int64_t pe, int64_t count);
+/* convey.h:328 convey_reset -- back to dormant; queued items are dropped. */
+int bale_convey_reset(bale_convey *c);
+/* convey.h:338 convey_free. */
+int bale_convey_free(bale_convey *c);
+/* Device-to-device copy of `bytes` on `stream` (to take items out of a span). */
+int bale_memcpy_d2d(void *dst, const void *src, int64_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

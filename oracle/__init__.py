@@ -57,6 +57,7 @@ def lib():
             "or_ig_agi_mt": (C.c_double, [P, P, I, P, C.c_int]),
             "or_batch_params": (None, [P, I, U, C.c_int, C.c_int, C.c_int, C.c_int]),
             "or_digest": (U, [P, I]),
+            "or_convey_deliver": (I, [P, P, I, I, I, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -172,6 +173,18 @@ def batch_params(count: int, seed: int, lo_tab=10, hi_tab=24, lo_ops=12, hi_ops=
     lib().or_batch_params(C.addressof(arr), count, seed, lo_tab, hi_tab, lo_ops, hi_ops)
     return [{"kind": x.kind, "dist": x.dist, "table_len": x.table_len, "nops": x.nops, "s": x.s,
              "seed": x.seed} for x in arr]
+
+
+def convey_deliver(items: np.ndarray, pes: np.ndarray, P: int) -> tuple[np.ndarray, np.ndarray, int]:
+    """(delivered items, starts[P+1], dropped) for items (n rows of bytes) sent to pes."""
+    items = np.ascontiguousarray(items)
+    pes = np.ascontiguousarray(pes, dtype=np.int64)
+    n = pes.size
+    ib = items.nbytes // n if n else 0
+    out = np.zeros_like(items)
+    starts = np.zeros(P + 1, dtype=np.int64)
+    dropped = lib().or_convey_deliver(_p(items), _p(pes), n, ib, P, _p(out), _p(starts))
+    return out[: starts[P]], starts, dropped
 
 
 def digest(v: np.ndarray) -> int:

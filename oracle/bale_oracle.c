@@ -251,6 +251,30 @@ int or_ig_conveyor(int64_t *tgt, const int64_t *pckindx, int64_t n,
   return 0;
 }
 
+/* ---- conveyor delivery (convey/narrative.h:135-137) ------------------- */
+int64_t or_convey_deliver(const void *items, const int64_t *pes, int64_t n, int64_t item_bytes,
+                          int64_t P, void *out, int64_t *starts) {
+  /* one FIFO per destination, filled in push order and emptied in order */
+  int64_t dropped = 0;
+  for (int64_t p = 0; p <= P; p++) starts[p] = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (pes[i] < 0 || pes[i] >= P) dropped++;
+    else starts[pes[i] + 1]++;
+  }
+  for (int64_t p = 0; p < P; p++) starts[p + 1] += starts[p];
+  int64_t *tail = (int64_t *)malloc((size_t)(P > 0 ? P : 1) * sizeof(int64_t));
+  if (!tail) return -1;
+  for (int64_t p = 0; p < P; p++) tail[p] = starts[p];
+  const unsigned char *src = (const unsigned char *)items;
+  unsigned char *dst = (unsigned char *)out;
+  for (int64_t i = 0; i < n; i++) {
+    if (pes[i] < 0 || pes[i] >= P) continue;
+    memcpy(dst + tail[pes[i]]++ * item_bytes, src + i * item_bytes, (size_t)item_bytes);
+  }
+  free(tail);
+  return dropped;
+}
+
 /* ---- timed multi-thread CPU baseline ---------------------------------- */
 typedef struct mt_arg {
   const int64_t *index;

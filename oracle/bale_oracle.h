@@ -104,6 +104,17 @@ typedef struct or_instance {
 void or_batch_params(or_instance *out, int64_t count, uint64_t seed,
                      int lo_tab, int hi_tab, int lo_ops, int hi_ops);
 
+/* ---------------- conveyor delivery ---------------------------------- */
+/* One source pushing n items (item_bytes each) to P destinations: every
+ * destination receives the items addressed to it in push order
+ * (convey/narrative.h:135-137; convey_push / convey_pull,
+ * convey/convey.h:237,265).  out = the destinations' items one after another
+ * (destination 0 first), starts[p] = first item of destination p, starts[P] =
+ * items delivered.  Items naming a destination outside [0,P) are dropped;
+ * returns how many. */
+int64_t or_convey_deliver(const void *items, const int64_t *pes, int64_t n, int64_t item_bytes,
+                          int64_t P, void *out, int64_t *starts);
+
 /* Order-independent 64-bit digest of an int64 array (sum of mix(v_i ^ i)). */
 uint64_t or_digest(const int64_t *v, int64_t n);
 

@@ -275,3 +275,48 @@ def test_digest_is_position_sensitive(orc):
     b[[3, 4]] = b[[4, 3]]
     assert orc.digest(a) != orc.digest(b)
     assert orc.digest(a) == orc.digest(a.copy())
+
+
+# ---- conveyor delivery ---------------------------------------------------------------
+def _fifo_brute(items, pes, P):
+    """Per-destination FIFO queues, filled in push order (textbook restatement)."""
+    q = [[] for _ in range(P)]
+    dropped = 0
+    for row, p in zip(items, pes):
+        if 0 <= p < P:
+            q[p].append(bytes(row))
+        else:
+            dropped += 1
+    return q, dropped
+
+
+@pytest.mark.parametrize("P,item_bytes", [(1, 8), (7, 16), (64, 4), (256, 32)])
+def test_convey_deliver_is_stable_partition(orc, P, item_bytes):
+    rng = np.random.default_rng(P * 100 + item_bytes)
+    n = 20_000
+    items = rng.integers(0, 256, size=(n, item_bytes), dtype=np.uint8)
+    pes = rng.integers(-2, P + 2, size=n).astype(np.int64)
+    out, starts, dropped = orc.convey_deliver(items, pes, P)
+    ok = (pes >= 0) & (pes < P)
+    assert dropped == int((~ok).sum())
+    # library routine: a stable sort of the valid items by destination
+    order = np.argsort(pes[ok], kind="stable")
+    assert np.array_equal(out, items[ok][order])
+    assert np.array_equal(starts, np.concatenate([[0], np.cumsum(np.bincount(pes[ok], minlength=P))]))
+
+
+def test_convey_deliver_brute_force(orc):
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        P = int(rng.integers(1, 9))
+        n = int(rng.integers(0, 40))
+        items = rng.integers(0, 256, size=(max(n, 1), 8), dtype=np.uint8)[:n]
+        pes = rng.integers(-1, P + 1, size=n).astype(np.int64)
+        if n == 0:
+            continue
+        out, starts, dropped = orc.convey_deliver(items, pes, P)
+        q, d2 = _fifo_brute(items, pes, P)
+        assert dropped == d2
+        for p in range(P):
+            got = [bytes(r) for r in out[starts[p]:starts[p + 1]]]
+            assert got == q[p], (trial, p)
