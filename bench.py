@@ -265,6 +265,51 @@ def load_traffic(name: str):
     return None
 
 
+def bench_convey(B, torch, dev, stream, n=1 << 26, P=64, reps=5):
+    """Device conveyor: push 2^26 8-byte items to 64 virtual PEs and deliver
+    them (advance), wall time of push + advance with the GPU idle before."""
+    from bale_old_amd.convey import Conveyor, DONE
+    items = torch.empty(n, dtype=torch.int64, device=dev)
+    B.fill_splitmix(items)
+    pes = torch.empty(n, dtype=torch.int64, device=dev)
+    B.gen_index(pes, P, seed=6, stream=stream)
+    want = torch.bincount(pes, minlength=P).cpu().tolist()
+    c = Conveyor(P, n, 8)
+    c.begin()
+    ts = []
+    for rep in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        assert c.push(items, pes, stream)
+        c.advance(False, stream)
+        t1 = time.perf_counter()
+        got = [c.pull_span(p)[1] for p in range(P)]
+        assert got == want, "conveyor spans do not match the PE counts"
+        if rep:
+            ts.append(t1 - t0)
+    assert c.advance(True, stream) == DONE
+    c.free()
+    t = median(ts)
+    return {"n": n, "pes": P, "item_bytes": 8, "median_s": t, "Gitems_per_s": n / t / 1e9,
+            "note": "push (device copy of items and PE numbers) + advance (count, offsets, "
+                    "stable delivery, span readback)"}
+
+
+def cpu_convey(n=1 << 24, P=64):
+    """The oracle's FIFO delivery of the same kind of input, one host thread."""
+    import numpy as np
+    import oracle
+    rng = np.random.default_rng(6)
+    items = rng.integers(0, 1 << 62, size=n, dtype=np.int64)
+    pes = rng.integers(0, P, size=n, dtype=np.int64)
+    oracle.convey_deliver(items, pes, P)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        oracle.convey_deliver(items, pes, P)
+    t = (time.perf_counter() - t0) / 3
+    return {"n": n, "pes": P, "Gitems_per_s": n / t / 1e9, "cores": 1, "kind": "port"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,6 +378,11 @@ def main() -> int:
             log(f"{name}: {r['GUPS']:.2f} GUPS cfg={r['cfg']}")
             torch.cuda.empty_cache()
         extra["histo"] = histo
+        log("conveyor 2^26 x 8 B -> 64 PEs ...")
+        extra["convey"] = bench_convey(B, torch, dev, stream)
+        if not args.no_cpu:
+            extra["convey"]["cpu_baseline"] = cpu_convey()
+        log(f"conveyor: {extra['convey']}")
 
     if not args.no_batch:
         from bale_old_amd import selector as S
