@@ -209,9 +209,30 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P,
 // for one sample and adds its gradient into LDS, then the workgroup applies an
 // Adam update.  Loss: sum over the three heads of softmax cross-entropy
 // (label < 0 means "no label" for that head).
+// Candidates of one training instance, for the cost-aware loss: the three
+// head classes of each candidate the tuner timed and its time over the best
+// one (>= 1); s < 0 ends the list.
+struct TrainCand {
+  float s, v, t, r;
+};
+constexpr float kCostSharpness = 8.f;  // (t_best / t)^8: 10% slower weighs 0.47, 50% slower 0.04
+
+// Training loss.  cands == nullptr: softmax cross-entropy of each head
+// against the tuner's argmin (labels).  Otherwise a cross-entropy over the
+// instance's timed candidates: a candidate's score is the sum of its heads'
+// log-probabilities (the rule inference uses), q = softmax of the scores over
+// the candidates, and the target weights each candidate by r^-kCostSharpness
+// (normalised), so near-best candidates share the target and slow ones get
+// almost none: loss = -sum_j w_j log q_j.  Its gradient with respect to head
+// h's logit m is the sum over candidates j with class m in head h of
+// q_j - w_j (the heads' own softmax terms cancel because both q and w sum
+// to 1).  Unlike the expected relative time sum_j q_j r_j, whose gradient
+// vanishes once q settles on any one candidate, this keeps pulling toward
+// the fast ones.
 __global__ __launch_bounds__(256) void k_mlp_train(float* __restrict__ P, float* __restrict__ adam,
                                                     const float* __restrict__ feat,
                                                     const int32_t* __restrict__ labels,
+                                                    const TrainCand* __restrict__ cands, int maxc,
                                                     int64_t count, int steps, int batch, float lr,
                                                     uint64_t seed, float* __restrict__ loss_out) {
   __shared__ float g[kParams];
@@ -234,6 +255,42 @@ __global__ __launch_bounds__(256) void k_mlp_train(float* __restrict__ P, float*
       float x[kFeat], h1[kHid], h2[kHid], out[kOut], dout[kOut];
       for (int k = 0; k < kFeat; k++) x[k] = feat[i * kFeat + k];
       mlp_forward(sp, x, h1, h2, out);
+      if (cands) {
+        float lp[kOut];
+        for (int hd = 0; hd < 3; hd++) {
+          int o0 = kHeadOff[hd], L = kHeadLen[hd];
+          float mx = out[o0];
+          for (int o = 1; o < L; o++) mx = fmaxf(mx, out[o0 + o]);
+          float z = 0.f;
+          for (int o = 0; o < L; o++) z += expf(out[o0 + o] - mx);
+          const float lz = logf(z) + mx;
+          for (int o = 0; o < L; o++) lp[o0 + o] = out[o0 + o] - lz;
+        }
+        const TrainCand* ci = cands + i * maxc;
+        float smax = -3.4e38f;
+        int nc = 0;
+        for (; nc < maxc && ci[nc].s >= 0.f; nc++) {
+          const float sc = lp[(int)ci[nc].s] + lp[3 + (int)ci[nc].v] + lp[7 + (int)ci[nc].t];
+          smax = fmaxf(smax, sc);
+        }
+        float z = 0.f, wz = 0.f;
+        for (int j = 0; j < nc; j++) {
+          z += expf(lp[(int)ci[j].s] + lp[3 + (int)ci[j].v] + lp[7 + (int)ci[j].t] - smax);
+          wz += powf(fmaxf(ci[j].r, 1.f), -kCostSharpness);
+        }
+        const float lz = logf(z) + smax;
+        for (int o = 0; o < kOut; o++) dout[o] = 0.f;
+        for (int j = 0; j < nc; j++) {
+          const float sc = lp[(int)ci[j].s] + lp[3 + (int)ci[j].v] + lp[7 + (int)ci[j].t];
+          const float qj = expf(sc - lz);
+          const float wj = powf(fmaxf(ci[j].r, 1.f), -kCostSharpness) / wz;
+          const float gj = qj - wj;
+          dout[(int)ci[j].s] += gj;
+          dout[3 + (int)ci[j].v] += gj;
+          dout[7 + (int)ci[j].t] += gj;
+          lsum += wj * (lz - sc);
+        }
+      } else
       for (int hd = 0; hd < 3; hd++) {
         int lab = labels[i * 3 + hd];
         int o0 = kHeadOff[hd], L = kHeadLen[hd];
@@ -303,6 +360,53 @@ __global__ __launch_bounds__(256) void k_mlp_train(float* __restrict__ P, float*
   for (int k = t; k < kParams; k += T) P[k] = sp[k];
 }
 
+// Mean and standard deviation of each feature over count rows (one
+// workgroup), written to stats[0..kFeat) and stats[kFeat..2 kFeat), and the
+// standardised copy of the features to out.
+__global__ __launch_bounds__(256) void k_feat_standardize(const float* __restrict__ feat,
+                                                           int64_t count, float* __restrict__ stats,
+                                                           float* __restrict__ out) {
+  __shared__ float red[8];
+  __shared__ float mu[kFeat], sd[kFeat];
+  for (int k = 0; k < kFeat; k++) {
+    float a = 0.f;
+    for (int64_t i = threadIdx.x; i < count; i += blockDim.x) a += feat[i * kFeat + k];
+    const float m = block_sum(a, red) / (float)count;
+    float b = 0.f;
+    for (int64_t i = threadIdx.x; i < count; i += blockDim.x) {
+      const float d = feat[i * kFeat + k] - m;
+      b += d * d;
+    }
+    const float v = block_sum(b, red) / (float)count;
+    if (threadIdx.x == 0) {
+      mu[k] = m;
+      sd[k] = sqrtf(v) + 1e-6f;
+      stats[k] = m;
+      stats[kFeat + k] = sqrtf(v) + 1e-6f;
+    }
+  }
+  __syncthreads();
+  for (int64_t q = threadIdx.x; q < count * kFeat; q += blockDim.x) {
+    const int k = (int)(q % kFeat);
+    out[q] = (feat[q] - mu[k]) / sd[k];
+  }
+}
+
+// Fold x' = (x - mu) / sd into the first layer: W1 <- W1 / sd,
+// b1 <- b1 - W1 mu / sd, so inference takes the raw features.
+__global__ __launch_bounds__(64) void k_mlp_fold(float* __restrict__ P,
+                                                 const float* __restrict__ stats) {
+  const int j = threadIdx.x;
+  if (j >= kHid) return;
+  float shift = 0.f;
+  for (int k = 0; k < kFeat; k++) {
+    const float w = P[kW1 + j * kFeat + k] / stats[kFeat + k];
+    P[kW1 + j * kFeat + k] = w;
+    shift += w * stats[k];
+  }
+  P[kB1 + j] -= shift;
+}
+
 }  // namespace bale
 
 using namespace bale;
@@ -361,7 +465,27 @@ int bale_mlp_train(float* params, float* adam_state, const float* features, cons
   if (count <= 0 || steps < 0 || batch <= 0 || !params || !adam_state || !features || !labels)
     return bale_set_error(BALE_ERR_ARG, "bad training arguments");
   hipLaunchKernelGGL(k_mlp_train, dim3(1), dim3(256), 0, (hipStream_t)stream, params, adam_state,
-                     features, labels, count, steps, batch, lr, seed, loss);
+                     features, labels, (const TrainCand*)nullptr, 0, count, steps, batch, lr, seed,
+                     loss);
+  int rc = (int)hipGetLastError();
+  return rc ? bale_set_error(BALE_ERR_HIP, "mlp train: %s", hipGetErrorString((hipError_t)rc)) : BALE_OK;
+}
+
+int bale_mlp_train_cost(float* params, float* adam_state, const float* features,
+                        const float* cands, int maxc, int64_t count, int steps, int batch,
+                        float lr, uint64_t seed, float* loss, float* scratch, void* stream) {
+  if (count <= 0 || steps < 0 || batch <= 0 || maxc <= 0 || !params || !adam_state || !features ||
+      !cands || !scratch)
+    return bale_set_error(BALE_ERR_ARG, "bad training arguments");
+  hipStream_t s = (hipStream_t)stream;
+  // scratch: 2 kFeat floats of statistics, then the standardised features
+  float* stats = scratch;
+  float* norm = scratch + 2 * kFeat;
+  hipLaunchKernelGGL(k_feat_standardize, dim3(1), dim3(256), 0, s, features, count, stats, norm);
+  hipLaunchKernelGGL(k_mlp_train, dim3(1), dim3(256), 0, s, params, adam_state, norm,
+                     (const int32_t*)nullptr, (const TrainCand*)cands, maxc, count, steps, batch,
+                     lr, seed, loss);
+  hipLaunchKernelGGL(k_mlp_fold, dim3(1), dim3(64), 0, s, params, (const float*)stats);
   int rc = (int)hipGetLastError();
   return rc ? bale_set_error(BALE_ERR_HIP, "mlp train: %s", hipGetErrorString((hipError_t)rc)) : BALE_OK;
 }

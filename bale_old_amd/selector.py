@@ -1,7 +1,7 @@
 """Learned strategy selector: sketch features -> fp32 MLP -> per-instance config.
 
 The sketch, inference and training run on the GPU in libbale_hip.so
-(bale_sketch, bale_mlp_infer, bale_mlp_train).  This module maps between
+(bale_sketch, bale_mlp_infer, bale_mlp_train, bale_mlp_train_cost).  This module maps between
 tuner candidates and the MLP's three heads, trains from the exhaustive
 tuner's timings, and stores / loads the weights.
 
@@ -40,6 +40,9 @@ def _proto():
     L.bale_mlp_infer.argtypes = [VP, VP, I64, VP, VP, VP]
     L.bale_mlp_train.restype = C.c_int
     L.bale_mlp_train.argtypes = [VP, VP, VP, VP, I64, C.c_int, C.c_int, C.c_float, C.c_uint64, VP, VP]
+    L.bale_mlp_train_cost.restype = C.c_int
+    L.bale_mlp_train_cost.argtypes = [VP, VP, VP, VP, C.c_int, I64, C.c_int, C.c_int, C.c_float,
+                                      C.c_uint64, VP, VP, VP]
     L._sel_proto = True
     return L
 
@@ -92,6 +95,55 @@ def train(feats, labels, *, steps=3000, batch=512, lr=3e-3, seed=7, params=None,
     check(_proto().bale_mlp_train(P.data_ptr(), adam.data_ptr(), feats.contiguous().data_ptr(),
                                   lab.data_ptr(), feats.shape[0], steps, batch, lr, seed,
                                   loss.data_ptr(), _stream(stream)))
+    torch.cuda.synchronize()
+    return P, loss.cpu()
+
+
+def cand_table(kinds, tlens, costs):
+    """(count x maxc x 4 float tensor, maxc): for each instance the head classes
+    of every candidate the tuner timed and its time over the instance's best;
+    rows past an instance's candidates have strategy -1."""
+    import torch
+    from .mix import candidates
+    cache = {}
+    rows = []
+    maxc = 1
+    for kind, tl, cs in zip(kinds, tlens, costs):
+        key = (kind, tl)
+        if key not in cache:
+            cache[key] = [labels_of(c) for c in candidates(kind, tl)]
+        labs = cache[key]
+        valid = [x for x in cs[:len(labs)] if x > 0]
+        best = min(valid) if valid else 1.0
+        row = [(a, b, c, x / best) for (a, b, c), x in zip(labs, cs) if x > 0]
+        rows.append(row)
+        maxc = max(maxc, len(row))
+    T = torch.full((len(rows), maxc, 4), -1.0, dtype=torch.float32)
+    for i, row in enumerate(rows):
+        if row:
+            T[i, :len(row)] = torch.tensor(row, dtype=torch.float32)
+    return T, maxc
+
+
+def train_cost(feats, cands, maxc, *, steps=4000, batch=512, lr=3e-3, seed=7, params=None,
+               stream=None):
+    """Train on the tuner's timings of every candidate (GPU): cross-entropy of
+    the choice over an instance's candidates against targets weighted by
+    (best / time)^8.  Features are standardised on the device and the
+    standardisation folded into the first layer.  Returns (params, per-step
+    loss)."""
+    import torch
+    from . import _stream
+    dev = feats.device
+    P = init_params(seed, dev) if params is None else params.clone()
+    adam = torch.zeros(2 * P.numel(), dtype=torch.float32, device=dev)
+    loss = torch.zeros(max(steps, 1), dtype=torch.float32, device=dev)
+    n = feats.shape[0]
+    scratch = torch.empty(2 * N_FEAT + n * N_FEAT, dtype=torch.float32, device=dev)
+    cd = cands.to(device=dev, dtype=torch.float32).contiguous()
+    check(_proto().bale_mlp_train_cost(P.data_ptr(), adam.data_ptr(), feats.contiguous().data_ptr(),
+                                       cd.data_ptr(), maxc, n, steps, batch, lr, seed,
+                                       loss.data_ptr(), scratch.data_ptr(), _stream(stream)))
     torch.cuda.synchronize()
     return P, loss.cpu()
 

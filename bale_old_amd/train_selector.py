@@ -5,7 +5,11 @@
 Training instances come from mixes with seeds 1000, 1001, ... (disjoint from
 the benchmark mixes, seeds 4 and 5), generated and tuned in chunks so that at
 most one chunk's tables are resident.  For each instance the sketch features
-are computed on the GPU and the tuner's fastest candidate is the label.
+are computed on the GPU and every candidate's timing is kept.  The default
+loss (--loss cost) is the cross-entropy of the choice over the instance's
+timed candidates against targets weighted by (best / time)^8; --loss ce trains
+on the fastest candidate's classes instead.  --from-dump retrains from a
+previous run's saved timings.
 """
 from __future__ import annotations
 
@@ -58,6 +62,9 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=str(S.WEIGHTS))
     ap.add_argument("--dump", default=None, help="also save features/labels/costs (torch.save)")
+    ap.add_argument("--from-dump", default=None,
+                    help="train from a --dump file instead of collecting (same seeds and chunks)")
+    ap.add_argument("--loss", choices=("cost", "ce"), default="cost")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -65,12 +72,28 @@ def main(argv=None) -> int:
 
     dev = torch.device("cuda:0")
     lib().bale_hip_init(0)
-    feats, labels, costs, kinds, tlens, nops = collect(a.instances, a.chunk, dev, a.min_ops, log=log)
+    if a.from_dump:
+        d = torch.load(a.from_dump, weights_only=True)
+        feats, labels, costs, kinds = d["features"], d["labels"], d["costs"], d["kinds"]
+        params, seed = [], 1000
+        while len(params) < feats.shape[0]:
+            params += M.mix_params(min(a.chunk, feats.shape[0] - len(params)), seed)
+            seed += 1
+        if [p.kind for p in params] != list(kinds):
+            raise SystemExit(f"{a.from_dump}: instances do not match seeds 1000.. in chunks of {a.chunk}")
+        tlens = [p.table_len for p in params]
+        nops = [p.nops for p in params]
+    else:
+        feats, labels, costs, kinds, tlens, nops = collect(a.instances, a.chunk, dev, a.min_ops, log=log)
     n = feats.shape[0]
     n_val = max(1, n // 10)
     tr_f, tr_l = feats[n_val:].to(dev), labels[n_val:].to(dev)
     va_f, va_l = feats[:n_val].to(dev), labels[:n_val]
-    P, loss = S.train(tr_f, tr_l, steps=a.steps, batch=a.batch, lr=a.lr)
+    if a.loss == "cost":
+        cands, maxc = S.cand_table(kinds[n_val:], tlens[n_val:], costs[n_val:])
+        P, loss = S.train_cost(tr_f, cands, maxc, steps=a.steps, batch=a.batch, lr=a.lr)
+    else:
+        P, loss = S.train(tr_f, tr_l, steps=a.steps, batch=a.batch, lr=a.lr)
     _, cls = S.infer(P, va_f)
     acc = (cls.cpu() == va_l).float().mean(dim=0).tolist()
     # regret on the validation split: sum of chosen candidates' costs over the
@@ -89,13 +112,15 @@ def main(argv=None) -> int:
         chosen_t += row[k] * nops[i]
         best_t += min(valid) * nops[i]
     meta = {"instances": n, "validation": n_val, "steps": a.steps, "batch": a.batch, "lr": a.lr,
+            "loss": a.loss,
             "final_loss": float(loss[-50:].mean()), "val_head_accuracy": acc,
             "val_time_vs_exhaustive": chosen_t / best_t if best_t else None,
             "label_counts": {h: torch.bincount(labels[:, h].long(), minlength=4).tolist()
                              for h in range(3)}}
     S.save(P, a.out, meta)
     if a.dump:
-        torch.save({"features": feats, "labels": labels, "costs": costs, "kinds": kinds}, a.dump)
+        torch.save({"features": feats, "labels": labels, "costs": costs, "kinds": kinds,
+                    "tlens": tlens, "nops": nops}, a.dump)
     print(json.dumps(meta))
     return 0
 

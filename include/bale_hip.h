@@ -215,6 +215,33 @@ int bale_probe_random_read(const int64_t *table, int64_t n, int64_t nreads,
 int bale_probe_random_read_policy(const int64_t *table, int64_t n, int64_t nreads,
                                   int64_t *out, int policy, void *stream);
 
+/* ---- learned selector ----------------------------------------------------
+ * 12 sketch features per instance and an fp32 MLP 12 -> 64 -> 64 -> 10
+ * (3 strategy, 4 variant, 3 task-size logits); all arrays device memory. */
+int bale_hip_device_info(int *cus, int *lds_per_block);
+int bale_sketch_feature_count(void);
+int bale_mlp_param_count(void);
+/* features[count][12] of the instances' index distributions. */
+int bale_sketch(const bale_instance *inst, int64_t count, float *features, void *stream);
+/* logits[count][10] and per-head argmax classes[count][3]. */
+int bale_mlp_infer(const float *params, const float *features, int64_t count, float *logits,
+                   int32_t *classes, void *stream);
+/* Minibatch Adam on per-head cross-entropy against labels[count][3]
+ * (adam_state: 2 x param_count zeros; loss[steps] per-step mean loss). */
+int bale_mlp_train(float *params, float *adam_state, const float *features, const int32_t *labels,
+                   int64_t count, int steps, int batch, float lr, uint64_t seed, float *loss,
+                   void *stream);
+/* Minibatch Adam on a cost-weighted choice loss: cands[count][maxc][4] holds each
+ * timed candidate's (strategy, variant, task) classes and its time over the
+ * instance's best (strategy < 0 ends a list); the loss is the cross-entropy of the
+ * softmax over the candidates' summed head log-probabilities against targets
+ * proportional to (best / time)^8.  Features are standardised on the
+ * device (scratch: 2 x 12 + count x 12 floats) and the standardisation is folded
+ * into the first layer, so the trained params take raw features. */
+int bale_mlp_train_cost(float *params, float *adam_state, const float *features,
+                        const float *cands, int maxc, int64_t count, int steps, int batch,
+                        float lr, uint64_t seed, float *loss, float *scratch, void *stream);
+
 /* ---- device conveyor ---------------------------------------------------
  * The conveyor interface (convey/convey.h:184-338) as a bulk service on one
  * GPU: items go to up to 256 virtual PEs, and each PE receives its items in

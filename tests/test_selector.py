@@ -102,3 +102,53 @@ def test_weights_roundtrip(dev, tmp_path):
     Q, meta = S.load(dev, tmp_path / "w.bin")
     assert meta == {"note": "test"}
     assert bool((P == Q).all())
+
+
+def test_cost_training_folds_standardisation(dev):
+    """steps=0: the returned parameters are the seeded initialisation with the
+    feature standardisation folded into the first layer, so raw features
+    through them equal standardised features through the initialisation."""
+    import torch
+    g = torch.Generator().manual_seed(1)
+    x = torch.rand((500, 12), generator=g) * torch.linspace(0.5, 30, 12) + torch.linspace(-3, 3, 12)
+    cands = torch.tensor([[[0, 0, 0, 1.0], [1, 2, 1, 1.5]]] * 500, dtype=torch.float32)
+    P, _ = S.train_cost(x.to(dev), cands, 2, steps=0)
+    P0 = S.init_params(7, dev).cpu().numpy().astype(np.float64)
+    xs = x.numpy().astype(np.float64)
+    z = (xs - xs.mean(0)) / (xs.std(0) + 1e-6)
+    want = np_forward(P0, z)
+    got = np_forward(P.cpu().numpy().astype(np.float64), xs)
+    np.testing.assert_allclose(got, want, rtol=2e-3, atol=2e-3)
+
+
+def test_cost_training_minimises_expected_time(dev):
+    """Candidates whose relative time depends on the features: training on the
+    expected relative time must pick near-best candidates, including where the
+    best one differs from head to head."""
+    import torch
+    g = torch.Generator().manual_seed(2)
+    n = 4000
+    x = torch.rand((n, 12), generator=g)
+    rows = []
+    for i in range(n):
+        a, b = float(x[i, 0]), float(x[i, 1])
+        # (strategy, variant, task, relative time)
+        r = [(0, 0, 1, 1.0 + 2 * a), (1, 1, 1, 1.0 + 2 * (1 - a)), (2, 2, 0, 1.0 + 3 * abs(b - 0.5)),
+             (2, 3, 2, 1.2)]
+        best = min(t for *_, t in r)
+        rows.append([(s, v, t, tt / best) for s, v, t, tt in r])
+    cands = torch.tensor(rows, dtype=torch.float32)
+    P, loss = S.train_cost(x.to(dev), cands, 4, steps=1500, batch=256, lr=5e-3)
+    assert loss[-100:].mean() < loss[:20].mean()
+    logits, _ = S.infer(P, x.to(dev))
+    lp = logits.cpu().double()
+    tot_chosen = tot_best = 0.0
+    for i in range(n):
+        sc = []
+        for s, v, t, r in rows[i]:
+            z = [torch.log_softmax(lp[i, o:o + k], 0) for o, k in S.HEADS]
+            sc.append(float(z[0][int(s)] + z[1][int(v)] + z[2][int(t)]))
+        j = int(np.argmax(sc))
+        tot_chosen += rows[i][j][3]
+        tot_best += 1.0
+    assert tot_chosen / tot_best < 1.05, tot_chosen / tot_best
