@@ -12,7 +12,9 @@
 //           (partition.h), then one LDS histogram per bucket segment, as
 //           exstack/conveyor route each update to its owner, which then applies
 //           it locally (apps/histo_src/histo_exstack.upc:64-76,
-//           histo_conveyor.upc:72-82).
+//           histo_conveyor.upc:72-82).  The paged plan reads the index once:
+//           full send buffers become pages of the workgroup's own pool, and a
+//           small pass lists each bucket's pages for the apply pass.
 // All three add into the existing counts (the drivers accumulate across
 // models, apps/histo_src/histo.upc:208-227).
 #include "bale_internal.h"
@@ -350,6 +352,185 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
   }
 }
 
+// ---- BUCKET, paged: the index is read once ---------------------------------
+// The scatter pass ships each full LDS buffer as a page of its chunk's own
+// pool (no count pass), a small pass lists every bucket's pages, and the apply
+// pass walks those pages.  The page depth follows the sample's verdict the
+// same way in all three kernels: S with the hot table on, S_cold without.
+template <bool HOT>
+__device__ inline int paged_depth(const uint32_t* flags, int S, int S_cold) {
+  const uint32_t fl = flags ? *flags : (kFlagHotBins | kFlagRunsSeen);
+  const bool h = HOT && (fl & kFlagHotBins);
+  return (!h && HOT && S_cold > 0) ? S_cold : S;
+}
+
+template <bool PACKED, bool HOT, bool RUNS, typename CT>
+__global__ __launch_bounds__(1024) void k_histo_scatter_paged(
+    const int64_t* __restrict__ idx, int64_t n, uint64_t nbins, int shift, int B, int S,
+    int64_t chunk, const uint32_t* __restrict__ hot_g, uint16_t* __restrict__ staged,
+    uint32_t* __restrict__ tags, uint32_t* __restrict__ rows, uint32_t* __restrict__ used,
+    CT* __restrict__ counts, ErrWords* err, const uint32_t* __restrict__ flags, int S_cold) {
+  const int Se = paged_depth<HOT>(flags, S, S_cold);
+  PagedShip pg{tags, (uint32_t)(blockIdx.x * part_pool_pages(chunk, B, Se)), rows, used, err};
+  part_scatter_adaptive<PACKED, HOT, RUNS, uint16_t>(
+      blockIdx.x, idx, n, nbins, shift, B, S, S_cold, chunk, nullptr, nullptr, hot_g,
+      [staged](uint32_t pos, uint16_t v) { staged[pos] = v; },
+      [](int64_t, uint64_t, uint64_t low) { return (uint16_t)low; },
+      [counts](uint32_t bin, uint32_t c) { atomic_add_dev(counts + bin, (CT)c); },
+      [](int64_t) {}, flags, pg);
+}
+
+// One workgroup per chunk: its pages, grouped by bucket.  plist[start[b] +
+// rows[c][b] + k] = k-th page of bucket b from chunk c (any order within a
+// chunk), with bit 31 set on a short page.
+template <bool HOT>
+__global__ __launch_bounds__(1024) void k_page_list(const uint32_t* __restrict__ tags,
+                                                    const uint32_t* __restrict__ rows,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ used, int B,
+                                                    int64_t chunk, int S, int S_cold,
+                                                    const uint32_t* __restrict__ flags,
+                                                    uint32_t* __restrict__ plist) {
+  extern __shared__ uint32_t cur[];
+  const int c = blockIdx.x;
+  const int Se = paged_depth<HOT>(flags, S, S_cold);
+  const uint32_t pool = (uint32_t)(c * part_pool_pages(chunk, B, Se));
+  for (int b = threadIdx.x; b < B; b += blockDim.x) cur[b] = start[b] + rows[(size_t)c * B + b];
+  __syncthreads();
+  const uint32_t np = used[c];
+  for (uint32_t k = threadIdx.x; k < np; k += blockDim.x) {
+    const uint32_t t = tags[pool + k];
+    const uint32_t short_page = (t >> 16) < (uint32_t)Se ? 0x80000000u : 0u;
+    plist[atomicAdd(&cur[t & 0xffffu], 1u)] = (pool + k) | short_page;
+  }
+}
+
+// lds_add8 / lds_add8_pk for the first `valid` (1..8) of the 8 offsets in x.
+template <bool RUNS, bool PK, typename CT>
+__device__ inline void lds_add_valid(uint32_t* sh, u32x4 x, uint32_t valid, CT* dst,
+                                     uint32_t* spilled) {
+  uint32_t v[8] = {x.x & 0xffff, x.x >> 16, x.y & 0xffff, x.y >> 16,
+                   x.z & 0xffff, x.z >> 16, x.w & 0xffff, x.w >> 16};
+  uint32_t inc[8], old[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) inc[i] = (uint32_t)i < valid ? 1u : 0u;
+  if (RUNS) {
+    // a run's count goes on its first entry; the others add nothing
+#pragma unroll
+    for (int i = 6; i >= 0; i--)
+      if (v[i] == v[i + 1]) {
+        inc[i] += inc[i + 1];
+        inc[i + 1] = 0;
+      }
+  }
+  if (!PK) {
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      if (inc[i]) atomicAdd(&sh[v[i]], inc[i]);
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    old[i] = inc[i] ? atomicAdd(&sh[v[i] >> 1], inc[i] << ((v[i] & 1u) * 16u)) : 0u;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    if (inc[i]) pk_fix(sh, v[i], inc[i], old[i], dst, spilled);
+}
+
+// Persistent: each workgroup takes segments (<= seg_pages pages of one bucket)
+// from a queue, histograms their entries in LDS and adds the tile into counts;
+// eight lanes read one 128-byte page of 64 entries.
+template <typename CT, bool HOT, bool RUNS, bool PK>
+__global__ __launch_bounds__(1024) void k_histo_apply_paged(
+    const uint16_t* __restrict__ staged, const uint32_t* __restrict__ tags,
+    const uint32_t* __restrict__ plist, const uint32_t* __restrict__ start,
+    const uint32_t* __restrict__ segstart, uint32_t* __restrict__ misc, int B, uint32_t seg_pages,
+    int shift, uint64_t nbins, int S, int S_cold, CT* __restrict__ counts,
+    const uint32_t* __restrict__ flags) {
+  extern __shared__ uint32_t sh[];
+  __shared__ uint32_t s_work[2];
+  __shared__ uint32_t s_spilled;  // PK: some counter of this segment spilled
+  const bool merge = RUNS && (!flags || (*flags & kFlagRunsSeen));
+  const uint32_t Se = (uint32_t)paged_depth<HOT>(flags, S, S_cold);
+  const uint32_t vpp = Se / 8u;  // 16-byte vectors per page (a power of two)
+  const int vsh = __builtin_ctz(vpp);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const uint32_t total = misc[0];
+  while (true) {
+    if (tid == 0) {
+      uint32_t w = atomicAdd(&misc[1], 1u);
+      uint32_t b = 0;
+      if (w < total) {
+        int lo = 0, hi = B - 1;  // last bucket whose first segment is <= w
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (segstart[mid] <= w) lo = mid; else hi = mid - 1;
+        }
+        b = (uint32_t)lo;
+      }
+      s_work[0] = w;
+      s_work[1] = b;
+      s_spilled = 0;
+    }
+    __syncthreads();
+    const uint32_t w = s_work[0], b = s_work[1];
+    if (w >= total) break;
+    const uint32_t nseg = segstart[b + 1] - segstart[b];
+    const uint32_t g0 = start[b] + (w - segstart[b]) * seg_pages;
+    const uint32_t g1 = min(start[b + 1], g0 + seg_pages);
+    const uint64_t lo_bin = (uint64_t)b << shift;
+    const uint32_t width = (uint32_t)min((uint64_t)1 << shift, nbins - lo_bin);
+    const uint32_t words = PK ? (width + 1u) >> 1 : width;
+    CT* dst = counts + lo_bin;
+    for (uint32_t k = tid; k < words; k += nthr) sh[k] = 0;
+    __syncthreads();
+    const uint32_t nv = (g1 - g0) << vsh;
+    constexpr int Q = 2;  // vectors per thread in flight
+    for (uint32_t v0 = tid; v0 < nv; v0 += Q * nthr) {
+      uint32_t ent[Q], cnt[Q];
+      u32x4 x[Q];
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const uint32_t v = v0 + q * nthr;
+        ent[q] = v < nv ? plist[g0 + (v >> vsh)] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const uint32_t v = v0 + q * nthr;
+        const uint32_t page = ent[q] & 0x7fffffffu;
+        cnt[q] = 0;
+        if (v < nv) {
+          cnt[q] = (ent[q] >> 31) ? tags[page] >> 16 : Se;
+          x[q] = __builtin_nontemporal_load((const u32x4*)(staged + (size_t)page * Se) +
+                                            (v & (vpp - 1u)));
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const uint32_t v = v0 + q * nthr;
+        const uint32_t first = (v & (vpp - 1u)) * 8u;
+        if (v < nv && cnt[q] > first) {
+          const uint32_t valid = min(8u, cnt[q] - first);
+          if (merge) lds_add_valid<true, PK>(sh, x[q], valid, dst, &s_spilled);
+          else lds_add_valid<false, PK>(sh, x[q], valid, dst, &s_spilled);
+        }
+      }
+    }
+    __syncthreads();
+    // a bucket with one segment is this workgroup's alone: plain adds, unless
+    // a spill already went to its counters atomically
+    const bool plain = nseg == 1 && !(PK && s_spilled);
+    for (uint32_t k = tid; k < width; k += nthr) {
+      uint32_t c = PK ? (sh[k >> 1] >> ((k & 1u) * 16u)) & 0xffffu : sh[k];
+      if (c) {
+        if (plain) dst[k] += (CT)c;
+        else atomic_add_dev(dst + k, (CT)c);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- launchers ------------------------------------------------------------
 static int norm_block(int b) { return (b == 256 || b == 512 || b == 1024) ? b : 256; }
 static int norm_unroll(int u) { return (u == 4 || u == 8 || u == 16) ? u : 8; }
@@ -543,20 +724,109 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   return (int)hipGetLastError();
 }
 
+// The paged plan: sketch, scatter into page pools, page counts prefixed over
+// chunks and buckets, page list, apply.  Returns 1 when the buffers would be
+// too shallow for 16-byte pages (the counted plan runs instead).
+template <bool PACKED, bool HOT, bool RUNS, typename CT>
+static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins,
+                              const bale_config& c, Device* d, hipStream_t s) {
+  int tb = c.tile_log2 > 0 ? c.tile_log2 : max_tile_bits(d);
+  const int tb_max = std::min(16, max_tile_bits(d) + 1);
+  if (tb > tb_max) tb = tb_max;
+  const bool pk = tb > max_tile_bits(d);
+  int64_t B = ceil_div(nbins, (int64_t)1 << tb);
+  if (B > kMaxBuckets) return (int)hipErrorInvalidValue;
+  int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
+  const size_t budget = (size_t)d->lds_per_block / (size_t)gpc;
+  int S = part_pick_depth((int)B, sizeof(uint16_t), budget, HOT);
+  const bool adaptive = HOT && c.hot == 2;
+  int S_cold = adaptive ? part_pick_depth((int)B, sizeof(uint16_t), budget, false) : 0;
+  const int s_lo = S_cold > 0 ? std::min(S, S_cold) : S;
+  if (s_lo < 8) return 1;
+  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
+  int64_t chunk = ceil_div(ceil_div(n, C), kChunkAlign) * kChunkAlign;
+  C = ceil_div(n, chunk);
+  // pool sizes for either depth: pages (tags, list) and entries (staging)
+  const int64_t pages = C * part_pool_pages(chunk, (int)B, s_lo);
+  const int64_t entries = std::max(C * part_pool_pages(chunk, (int)B, S) * S,
+                                   S_cold > 0 ? C * part_pool_pages(chunk, (int)B, S_cold) * S_cold : 0);
+  if (entries >= ((int64_t)1 << 32) || pages >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
+  const size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
+  const size_t tag_b = ((size_t)pages * 4 + 255) & ~(size_t)255;
+  const size_t used_b = ((size_t)C * 4 + 255) & ~(size_t)255;
+  const size_t stage_b = ((size_t)entries * 2 + 255) & ~(size_t)255;
+  void* ws = workspace(d, head + 2 * tag_b + used_b + stage_b, s);
+  if (!ws) return (int)hipErrorOutOfMemory;
+  PartLayout L = part_carve(ws, C, B);
+  uint32_t* tags = (uint32_t*)((char*)ws + head);
+  uint32_t* plist = (uint32_t*)((char*)ws + head + tag_b);
+  uint32_t* used = (uint32_t*)((char*)ws + head + 2 * tag_b);
+  uint16_t* staged = (uint16_t*)((char*)ws + head + 2 * tag_b + used_b);
+  uint32_t* flags = adaptive ? L.misc + (kMiscWords - 1) : nullptr;
+  if (HOT)
+    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins,
+                       L.hot, flags);
+  size_t lds = part_scatter_lds((int)B, S, sizeof(uint16_t), HOT);
+  if (S_cold > 0) lds = std::max(lds, part_scatter_lds((int)B, S_cold, sizeof(uint16_t), false));
+  allow_lds(k_histo_scatter_paged<PACKED, HOT, RUNS, CT>, lds);
+  hipLaunchKernelGGL((k_histo_scatter_paged<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024),
+                     lds, s, idx, n, (uint64_t)nbins, tb, (int)B, S, chunk, L.hot, staged, tags,
+                     L.chunk, used, counts, d->err, (const uint32_t*)flags, S_cold);
+  hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk, (int)C, (int)B,
+                     L.total);
+  // segments as in the counted plan, in pages of the hot-on depth
+  const int64_t seg = std::max<int64_t>({4096, ceil_div(n, (int64_t)d->cus * 2), (int64_t)4 << tb});
+  const uint32_t seg_pages = (uint32_t)ceil_div(seg, S);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, seg_pages, L.start,
+                     L.segstart, L.misc);
+  hipLaunchKernelGGL(k_page_list<HOT>, dim3((unsigned)C), dim3(1024), (size_t)B * 4, s,
+                     (const uint32_t*)tags, (const uint32_t*)L.chunk, (const uint32_t*)L.start,
+                     (const uint32_t*)used, (int)B, chunk, S, S_cold, (const uint32_t*)flags, plist);
+  size_t lds_apply = (size_t)(pk ? 2 : 4) << tb;
+  int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
+  if (pk) {
+    allow_lds(k_histo_apply_paged<CT, HOT, RUNS, true>, lds_apply);
+    hipLaunchKernelGGL((k_histo_apply_paged<CT, HOT, RUNS, true>), dim3((unsigned)groups),
+                       dim3(1024), lds_apply, s, (const uint16_t*)staged, (const uint32_t*)tags,
+                       (const uint32_t*)plist, (const uint32_t*)L.start, (const uint32_t*)L.segstart,
+                       L.misc, (int)B, seg_pages, tb, (uint64_t)nbins, S, S_cold, counts,
+                       (const uint32_t*)flags);
+  } else {
+    allow_lds(k_histo_apply_paged<CT, HOT, RUNS, false>, lds_apply);
+    hipLaunchKernelGGL((k_histo_apply_paged<CT, HOT, RUNS, false>), dim3((unsigned)groups),
+                       dim3(1024), lds_apply, s, (const uint16_t*)staged, (const uint32_t*)tags,
+                       (const uint32_t*)plist, (const uint32_t*)L.start, (const uint32_t*)L.segstart,
+                       L.misc, (int)B, seg_pages, tb, (uint64_t)nbins, S, S_cold, counts,
+                       (const uint32_t*)flags);
+  }
+  return (int)hipGetLastError();
+}
+
+// The paged plan reads the index once and is the faster one on every measured
+// shape (MI355X, 2^24..2^28 updates, uniform and Zipf); the counted plan runs
+// only where buffers are too shallow for pages.
+template <bool PACKED, bool HOT, bool RUNS, typename CT>
+static int bucket_any(const int64_t* idx, int64_t n, CT* counts, int64_t nbins,
+                      const bale_config& c, Device* d, hipStream_t s) {
+  int rc = bucket_paged_typed<PACKED, HOT, RUNS, CT>(idx, n, counts, nbins, c, d, s);
+  if (rc != 1) return rc;
+  return bucket_typed<PACKED, HOT, RUNS, CT>(idx, n, counts, nbins, c, d, s);
+}
+
 template <typename CT>
 static int bucket_dispatch(const int64_t* idx, int64_t n, CT* counts, int64_t nbins, bool packed,
                            const bale_config& c, Device* d, hipStream_t s) {
   const bool hot = c.hot != 0, runs = c.merge > 0 || c.hot == 2;
   const int sel = (packed ? 4 : 0) | (hot ? 2 : 0) | (runs ? 1 : 0);
   switch (sel) {
-    case 0: return bucket_typed<false, false, false, CT>(idx, n, counts, nbins, c, d, s);
-    case 1: return bucket_typed<false, false, true, CT>(idx, n, counts, nbins, c, d, s);
-    case 2: return bucket_typed<false, true, false, CT>(idx, n, counts, nbins, c, d, s);
-    case 3: return bucket_typed<false, true, true, CT>(idx, n, counts, nbins, c, d, s);
-    case 4: return bucket_typed<true, false, false, CT>(idx, n, counts, nbins, c, d, s);
-    case 5: return bucket_typed<true, false, true, CT>(idx, n, counts, nbins, c, d, s);
-    case 6: return bucket_typed<true, true, false, CT>(idx, n, counts, nbins, c, d, s);
-    default: return bucket_typed<true, true, true, CT>(idx, n, counts, nbins, c, d, s);
+    case 0: return bucket_any<false, false, false, CT>(idx, n, counts, nbins, c, d, s);
+    case 1: return bucket_any<false, false, true, CT>(idx, n, counts, nbins, c, d, s);
+    case 2: return bucket_any<false, true, false, CT>(idx, n, counts, nbins, c, d, s);
+    case 3: return bucket_any<false, true, true, CT>(idx, n, counts, nbins, c, d, s);
+    case 4: return bucket_any<true, false, false, CT>(idx, n, counts, nbins, c, d, s);
+    case 5: return bucket_any<true, false, true, CT>(idx, n, counts, nbins, c, d, s);
+    case 6: return bucket_any<true, true, false, CT>(idx, n, counts, nbins, c, d, s);
+    default: return bucket_any<true, true, true, CT>(idx, n, counts, nbins, c, d, s);
   }
 }
 

@@ -374,6 +374,8 @@ def main() -> int:
             log(f"{name} ...")
             r = bench_histo(B, torch, dev, hsteps, 2, stream, **kw)
             r["GUPS"] = r["n"] / r["median_s"] / 1e9
+            # algorithmic bytes: 8 B index + counter read-modify-write (16 B int64, 8 B int32)
+            r["alg_GBps"] = r["GUPS"] * (24 if kw["dtype"] == torch.int64 else 16)
             histo[name] = r
             log(f"{name}: {r['GUPS']:.2f} GUPS cfg={r['cfg']}")
             torch.cuda.empty_cache()

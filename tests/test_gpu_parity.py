@@ -126,6 +126,26 @@ def test_histo_packed_tile_spills(dev, case, dtype):
         assert np.array_equal(to_np(counts).astype(np.int64), want + 5), (case, merge, hot)
 
 
+@pytest.mark.parametrize("nbins,n,tile_log2,dist", [
+    ((1 << 21) - 5, 1 << 20, 9, 0),    # 4096 buckets: shallowest pages, mostly short ones
+    ((1 << 21) - 5, 1 << 20, 9, 1),
+    (1 << 16, 3000, 10, 0),            # one chunk, fewer updates than buckets x depth
+    ((1 << 24) + 1, 1 << 22, 16, 1),   # 2^16-bin buckets, packed tile, one short last bucket
+    (5_000_011, 1 << 22, 13, 0),       # uniform: the sample switches to the deeper cold pages
+])
+def test_histo_bucket_pages(dev, orc, nbins, n, tile_log2, dist):
+    """The bucketed plan ships LDS buffers as pages of per-chunk pools; short
+    pages, the deepest and shallowest page sizes, and the adaptive switch."""
+    idx = gen(dev, n, nbins, dist, 1.2, 0, seed=nbins ^ n)
+    want = np.zeros(nbins, np.int64)
+    orc.histo_agi(to_np(idx), want)
+    for hot, merge in ((0, 0), (1, 0), (2, 0), (0, 1)):
+        counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
+        cfg = B.config(B.BUCKET, block=1024, tile_log2=tile_log2, hot=hot, merge=merge)
+        assert B.histo(idx, counts, cfg=cfg) == 0
+        assert np.array_equal(to_np(counts), want), (hot, merge)
+
+
 def test_histo_packed_and_errors(dev, orc):
     nbins, n = 4096, 100_000
     idx = gen(dev, n, nbins, seed=9)
