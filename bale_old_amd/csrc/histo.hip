@@ -13,10 +13,13 @@
 //           exstack/conveyor route each update to its owner, which then applies
 //           it locally (apps/histo_src/histo_exstack.upc:64-76,
 //           histo_conveyor.upc:72-82).  The paged plan reads the index once:
-//           full send buffers become pages of the workgroup's own pool, and a
-//           small pass lists each bucket's pages for the apply pass.
+//           each bucket's send buffer is a ring in LDS whose completed halves
+//           are shipped as pages of the workgroup's own pool, and a small pass
+//           lists each bucket's pages for the apply pass.
 // All three add into the existing counts (the drivers accumulate across
 // models, apps/histo_src/histo.upc:208-227).
+#include <type_traits>
+
 #include "bale_internal.h"
 #include "partition.h"
 
@@ -267,6 +270,67 @@ __device__ inline void lds_add1_pk(uint32_t* sh, uint32_t bin, CT* dst, uint32_t
   pk_fix(sh, bin, 1u, old, dst, spilled);
 }
 
+// Add an LDS tile of counts (32-bit, or PK: two 16-bit per word) into
+// dst[0..width).  Not plain: a device atomic per nonzero count.  Plain (the
+// workgroup owns these bins): read-modify-write of four consecutive counters
+// per thread per step with four steps' loads in flight, so the counter traffic
+// streams instead of waiting on one load at a time.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+template <bool PK>
+__device__ inline uint32_t tile_count(const uint32_t* sh, uint32_t k) {
+  return PK ? (sh[k >> 1] >> ((k & 1u) * 16u)) & 0xffffu : sh[k];
+}
+// four consecutive counters as 16-byte vectors
+struct Cnt4_64 { i64x2 a, b; };
+__device__ inline Cnt4_64 load4(const int64_t* p) { return {*(const i64x2*)p, *(const i64x2*)(p + 2)}; }
+__device__ inline void store4(int64_t* p, Cnt4_64 v, const uint32_t* c) {
+  v.a.x += c[0]; v.a.y += c[1]; v.b.x += c[2]; v.b.y += c[3];
+  *(i64x2*)p = v.a;
+  *(i64x2*)(p + 2) = v.b;
+}
+__device__ inline i32x4 load4(const int32_t* p) { return *(const i32x4*)p; }
+__device__ inline void store4(int32_t* p, i32x4 v, const uint32_t* c) {
+  v.x += (int)c[0]; v.y += (int)c[1]; v.z += (int)c[2]; v.w += (int)c[3];
+  *(i32x4*)p = v;
+}
+template <typename CT, bool PK>
+__device__ inline void flush_tile(const uint32_t* sh, uint32_t width, CT* dst, bool plain, int tid,
+                                  int nthr) {
+  if (!plain) {
+    for (uint32_t k = tid; k < width; k += nthr) {
+      const uint32_t c = tile_count<PK>(sh, k);
+      if (c) atomic_add_dev(dst + k, (CT)c);
+    }
+    return;
+  }
+  constexpr int F = 4;
+  const uint32_t w4 = (((uintptr_t)dst & 15u) == 0) ? (width & ~3u) : 0u;
+  const uint32_t step = (uint32_t)nthr * 4u;
+  for (uint32_t g0 = (uint32_t)tid * 4u; g0 < w4; g0 += step * F) {
+    uint32_t c[F][4];
+    decltype(load4(dst)) v[F];
+    bool any[F];
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      const uint32_t k = g0 + f * step;
+#pragma unroll
+      for (int i = 0; i < 4; i++) c[f][i] = k < w4 ? tile_count<PK>(sh, k + i) : 0u;
+      any[f] = (c[f][0] | c[f][1] | c[f][2] | c[f][3]) != 0;
+    }
+    // all loads first, then the stores
+#pragma unroll
+    for (int f = 0; f < F; f++)
+      if (any[f]) v[f] = load4(dst + g0 + f * step);
+#pragma unroll
+    for (int f = 0; f < F; f++)
+      if (any[f]) store4(dst + g0 + f * step, v[f], c[f]);
+  }
+  for (uint32_t k = w4 + tid; k < width; k += nthr) {
+    const uint32_t c = tile_count<PK>(sh, k);
+    if (c) dst[k] += (CT)c;
+  }
+}
+
 // Persistent: each workgroup takes segments (<= seg_len entries of one bucket)
 // from a queue, histograms them in LDS and adds the tile into counts.  A bucket
 // with one segment is owned by one workgroup, so its flush is a plain add.
@@ -341,13 +405,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
     // a bucket with one segment is this workgroup's alone: plain adds, unless
     // a spill already went to its counters atomically
     const bool plain = nseg == 1 && !(PK && s_spilled);
-    for (uint32_t k = tid; k < width; k += nthr) {
-      uint32_t c = PK ? (sh[k >> 1] >> ((k & 1u) * 16u)) & 0xffffu : sh[k];
-      if (c) {
-        if (plain) dst[k] += (CT)c;
-        else atomic_add_dev(dst + k, (CT)c);
-      }
-    }
+    flush_tile<CT, PK>(sh, width, dst, plain, tid, nthr);
     __syncthreads();
   }
 }
@@ -364,25 +422,312 @@ __device__ inline int paged_depth(const uint32_t* flags, int S, int S_cold) {
   return (!h && HOT && S_cold > 0) ? S_cold : S;
 }
 
-template <bool PACKED, bool HOT, bool RUNS, typename CT>
-__global__ __launch_bounds__(1024) void k_histo_scatter_paged(
-    const int64_t* __restrict__ idx, int64_t n, uint64_t nbins, int shift, int B, int S,
-    int64_t chunk, const uint32_t* __restrict__ hot_g, uint16_t* __restrict__ staged,
-    uint32_t* __restrict__ tags, uint32_t* __restrict__ rows, uint32_t* __restrict__ used,
-    CT* __restrict__ counts, ErrWords* err, const uint32_t* __restrict__ flags, int S_cold) {
-  const int Se = paged_depth<HOT>(flags, S, S_cold);
-  PagedShip pg{tags, (uint32_t)(blockIdx.x * part_pool_pages(chunk, B, Se)), rows, used, err};
-  part_scatter_adaptive<PACKED, HOT, RUNS, uint16_t>(
-      blockIdx.x, idx, n, nbins, shift, B, S, S_cold, chunk, nullptr, nullptr, hot_g,
-      [staged](uint32_t pos, uint16_t v) { staged[pos] = v; },
-      [](int64_t, uint64_t, uint64_t low) { return (uint16_t)low; },
-      [counts](uint32_t bin, uint32_t c) { atomic_add_dev(counts + bin, (CT)c); },
-      [](int64_t) {}, flags, pg);
+// Paged scatter through ring buffers.  Each bucket has a ring of R = 2P
+// 16-bit offsets in LDS.  Slot s (the s-th update of this chunk to the bucket,
+// from an LDS counter) goes to ring position s mod R once s < sent + R (sent:
+// the bucket's entries already shipped); the slot that completes a half
+// (s mod P == P - 1) lists that half, and after a barrier the listed halves
+// are shipped as pages.  A round of 8192 updates brings about 8192 / B updates
+// to a bucket, so a slot rarely has to wait for a ship; one that does stays in
+// its thread's registers and is placed after the next ship.  Written slots are
+// always a prefix of each bucket's slots, so every ship frees the half that
+// the oldest waiting slot needs.
+// LDS: fill[B], sent[B], list[L], misc[8], sink[64], [hot keys, hot counts], ring[B][R].
+// Pages a chunk of `len` updates can fill with pages of P entries: full pages
+// plus one short page per bucket.
+__host__ __device__ inline int64_t part_pool_pages(int64_t len, int B, int P) {
+  return (len + P - 1) / P + B;
+}
+constexpr int kRingRound = kScatterU * 1024;  // updates per round (1024 threads)
+constexpr int kRingMisc = 8;                  // [0,1] list lengths, [2,3] waiting flags
+// Halves one phase can list: one per touched bucket, plus a second one only
+// for a bucket that took more than P slots in the phase.
+__host__ __device__ inline int ring_list_len(int B, int R) { return B + kRingRound / (R / 2); }
+__host__ __device__ inline size_t ring_scatter_lds(int B, int R, bool hot) {
+  size_t head = (size_t)(2 * B + ring_list_len(B, R) + kRingMisc + 64) * 4 +
+                (hot ? (size_t)2 * kHotSlots * 4 : 0);
+  return ((head + 15) & ~(size_t)15) + (size_t)B * R * 2;
+}
+// Deepest ring (R a power of two in [16, 256]) that fits `budget` bytes; 0 if none.
+static int ring_pick(int B, size_t budget, bool hot) {
+  int R = 256;
+  while (R > 16 && ring_scatter_lds(B, R, hot) > budget) R >>= 1;
+  return ring_scatter_lds(B, R, hot) <= budget ? R : 0;
 }
 
-// One workgroup per chunk: its pages, grouped by bucket.  plist[start[b] +
-// rows[c][b] + k] = k-th page of bucket b from chunk c (any order within a
-// chunk), with bit 31 set on a short page.
+template <bool PACKED, bool HOT, bool RUNS, typename CT>
+__device__ __attribute__((always_inline)) inline void ring_scatter_body(
+    int64_t c, const int64_t* __restrict__ idx, int64_t n, uint64_t limit, int shift, int B, int R,
+    int64_t chunk, const uint32_t* __restrict__ hot_g, uint16_t* __restrict__ out,
+    uint32_t* __restrict__ tags, uint32_t pool, uint32_t* __restrict__ rows,
+    uint32_t* __restrict__ used, CT* __restrict__ counts, ErrWords* err) {
+  constexpr int U = kScatterU;  // updates per thread per round, loaded as U/2 pairs
+  constexpr int U2 = U / 2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int L = ring_list_len(B, R);
+  uint32_t* fill = (uint32_t*)smem;
+  uint32_t* sent = fill + B;
+  uint32_t* list = sent + B;
+  uint32_t* misc = list + L;
+  uint32_t* sink = misc + kRingMisc;
+  uint32_t* hkeys = sink + 64;
+  uint32_t* hcnt = hkeys + kHotSlots;
+  const size_t head = (size_t)(2 * B + L + kRingMisc + 64) * 4 + (HOT ? (size_t)2 * kHotSlots * 4 : 0);
+  uint16_t* ring = (uint16_t*)(smem + ((head + 15) & ~(size_t)15));
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int lane = __lane_id();
+  const uint32_t P = (uint32_t)R >> 1, R1 = (uint32_t)R - 1u, P1 = P - 1u;
+  const int rsh = __builtin_ctz((uint32_t)R), psh = rsh - 1;
+  const uint32_t vpp = P / 8u;  // 16-byte pieces per page
+  const int vsh = __builtin_ctz(vpp);
+  for (int b = tid; b < B; b += nthr) {
+    fill[b] = 0;
+    sent[b] = 0;
+  }
+  if (HOT)
+    for (int k = tid; k < kHotSlots; k += nthr) {
+      hkeys[k] = hot_g[k];
+      hcnt[k] = 0;
+    }
+  if (tid < kRingMisc) misc[tid] = 0;
+  __syncthreads();
+  const uint32_t mask = (1u << shift) - 1u;
+  const int64_t lo = c * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  uint32_t bad_r = 0, bad_p = 0;
+  uint32_t pages = 0;  // full pages shipped (the same in every thread)
+  int p = 0;           // phase parity: list length misc[p], waiting flag misc[2 + p]
+
+  // One update: its bin (bins fit 32 bits), and whether it counts at all.
+  auto decode = [&](int64_t raw, bool valid, uint32_t& j) -> bool {
+    uint64_t v = (uint64_t)raw;
+    bool pe_ok = true;
+    if (PACKED) {
+      pe_ok = (v & 0xffffu) == 0;
+      v = (uint64_t)(raw >> 16);
+    }
+    const bool in = v < limit;
+    bad_p += (valid && !pe_ok) ? 1u : 0u;
+    bad_r += (valid && pe_ok && !in) ? 1u : 0u;
+    j = (uint32_t)v;
+    return valid && pe_ok && in;
+  };
+  // Count a hot bin in LDS or a long run with one device atomic; otherwise the
+  // update is placed (returns true) with its bucket and 16-bit entry.  Every
+  // lane of the wave calls it together (wave_runs).
+  auto classify = [&](bool ok, uint32_t j, uint32_t& bk, uint32_t& ent) -> bool {
+    int hs = -1;
+    if (HOT && ok) hs = hot_find(hkeys, j);
+    if (HOT) {
+      if (RUNS) {
+        uint32_t len;
+        const int h0 = wave_runs(hs >= 0 ? (uint32_t)hs : kHotEmpty, len);
+        if (hs >= 0 && lane == h0) atomicAdd(&hcnt[hs], len);
+      } else if (hs >= 0) {
+        atomicAdd(&hcnt[hs], 1u);
+      }
+    }
+    bool cd = ok && hs < 0;
+    if (RUNS) {
+      uint32_t len;
+      const int h0 = wave_runs(cd ? j : kHotEmpty, len);
+      if (cd && len >= kLongRun) {
+        if (lane == h0) atomic_add_dev(counts + j, (CT)len);
+        cd = false;
+      }
+    }
+    bk = cd ? j >> shift : 0u;
+    ent = j & mask;
+    return cd;
+  };
+  // Append the halves completed by this thread's slots (done bits) to the
+  // list: one LDS atomic per wave; ballots are recomputed, not kept.
+  auto list_append = [&](uint32_t done, const uint32_t* bk, const uint32_t* s) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) tot += (uint32_t)__popcll(__ballot((done >> u) & 1u));
+    if (tot == 0) return;  // the same in every lane
+    uint32_t b0 = 0;
+    if (lane == 0) b0 = atomicAdd(&misc[p], tot);
+    b0 = __builtin_amdgcn_readfirstlane(b0);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t bal = __ballot((done >> u) & 1u);
+      if ((done >> u) & 1u) {
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+        list[b0 + below] = bk[u] | (((s[u] >> psh) & 1u) << 16);
+      }
+      b0 += (uint32_t)__popcll(bal);
+    }
+  };
+  // Barrier, ship the listed halves, barrier.  Returns whether some thread
+  // still has waiting slots (`wait`: this thread has).
+  auto ship_phase = [&](bool wait) -> bool {
+    if (__ballot(wait) != 0 && lane == 0) misc[2 + p] = 1u;
+    __syncthreads();
+    const uint32_t nl = misc[p];
+    const bool waiting = misc[2 + p] != 0;
+    const uint32_t total = nl << vsh;
+    for (uint32_t v = (uint32_t)tid; v < total; v += (uint32_t)nthr) {
+      const uint32_t k = v >> vsh, q = v & (vpp - 1u);
+      const uint32_t e = list[k], b = e & 0xffffu, h = e >> 16;
+      const uint32_t page = pool + pages + k;
+      const u32x4 x = ((const u32x4*)(ring + ((size_t)b << rsh) + ((size_t)h << psh)))[q];
+      __builtin_nontemporal_store(x, (u32x4*)out + ((size_t)page << vsh) + q);
+      if (q == 0) {
+        tags[page] = b | (P << 16);
+        atomicAdd(&sent[b], P);
+      }
+    }
+    pages += nl;
+    // the other parity's length and flag were last read a phase ago, before
+    // the barrier that ended it, and are next written after the barrier below
+    if (tid == 0) {
+      misc[p ^ 1] = 0;
+      misc[2 + (p ^ 1)] = 0;
+    }
+    p ^= 1;
+    __syncthreads();
+    return waiting;
+  };
+  // One round: thread tid holds updates base + 2 (k * 1024 + tid) + {0, 1}.
+  // FULL: the whole round is inside the chunk (no bounds tests).
+  auto round = [&](auto full_c, int64_t base, const i64x2* raw) {
+    constexpr bool FULL = decltype(full_c)::value;
+    uint32_t bk[U], ent[U], s[U];
+    int rhead[U];
+    uint32_t cold = 0;
+#pragma unroll
+    for (int k = 0; k < U2; k++)
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int u = 2 * k + h;
+        const bool valid = FULL || base + 2 * (k * 1024 + tid) + h < hi;
+        uint32_t j;
+        const bool ok = decode(h ? raw[k].y : raw[k].x, valid, j);
+        cold |= (classify(ok, j, bk[u], ent[u]) ? 1u : 0u) << u;
+      }
+    // reserve slots: one LDS atomic per update (per run of a bucket with RUNS);
+    // lanes with nothing to place add to their own sink word
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool cd = (cold >> u) & 1u;
+      uint32_t len = 1;
+      rhead[u] = lane;
+      if (RUNS) rhead[u] = wave_runs(cd ? bk[u] : kHotEmpty, len);
+      s[u] = atomicAdd((cd && lane == rhead[u]) ? &fill[bk[u]] : &sink[lane], len);
+    }
+    uint32_t se[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) se[u] = sent[bk[u]];  // bucket 0 for updates not placed
+    uint32_t done = 0, wait = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (RUNS) s[u] = __shfl(s[u], rhead[u]) + (uint32_t)(lane - rhead[u]);
+      const bool cd = (cold >> u) & 1u;
+      const bool fits = cd && s[u] - se[u] <= R1;
+      if (fits) ring[(bk[u] << rsh) | (s[u] & R1)] = (uint16_t)ent[u];
+      done |= ((fits && (s[u] & P1) == P1) ? 1u : 0u) << u;
+      wait |= ((cd && !fits) ? 1u : 0u) << u;
+    }
+    list_append(done, bk, s);
+    bool waiting = ship_phase(wait != 0);
+    while (waiting) {
+      // slots whose half of the ring has been shipped since
+      done = 0;
+      uint32_t still = 0;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const bool w = (wait >> u) & 1u;
+        const bool fits = w && s[u] - sent[bk[u]] <= R1;
+        if (fits) ring[(bk[u] << rsh) | (s[u] & R1)] = (uint16_t)ent[u];
+        done |= ((fits && (s[u] & P1) == P1) ? 1u : 0u) << u;
+        still |= ((w && !fits) ? 1u : 0u) << u;
+      }
+      wait = still;
+      list_append(done, bk, s);
+      waiting = ship_phase(wait != 0);
+    }
+  };
+
+  // 16-byte index loads when the index is 16-byte aligned (rounds start at
+  // multiples of 8192 updates)
+  const bool vec = ((uintptr_t)idx & 15u) == 0;
+  auto load_round = [&](int64_t base, i64x2* dst) {
+    if (vec && base + kRingRound <= hi) {
+      const i64x2* rp = (const i64x2*)(idx + base) + tid;
+#pragma unroll
+      for (int k = 0; k < U2; k++) dst[k] = __builtin_nontemporal_load(rp + k * 1024);
+    } else {
+#pragma unroll
+      for (int k = 0; k < U2; k++) {
+        const int64_t e = base + 2 * (k * 1024 + tid);
+        dst[k].x = e < hi ? __builtin_nontemporal_load(idx + e) : 0;
+        dst[k].y = e + 1 < hi ? __builtin_nontemporal_load(idx + e + 1) : 0;
+      }
+    }
+  };
+  i64x2 nxt[U2];
+  if (lo < hi) load_round(lo, nxt);
+  for (int64_t base = lo; base < hi; base += kRingRound) {
+    i64x2 raw[U2];
+#pragma unroll
+    for (int k = 0; k < U2; k++) raw[k] = nxt[k];
+    if (base + kRingRound < hi) load_round(base + kRingRound, nxt);
+    if (base + kRingRound <= hi) round(std::true_type(), base, raw);
+    else round(std::false_type(), base, raw);
+  }
+  // each bucket's unshipped slots (fewer than P, in one half) become short
+  // page `pages + b`; the last ship phase ended with a barrier
+  const uint32_t total = (uint32_t)B << vsh;
+  for (uint32_t v = (uint32_t)tid; v < total; v += (uint32_t)nthr) {
+    const uint32_t b = v >> vsh, q = v & (vpp - 1u);
+    const uint32_t sd = sent[b], cnt = fill[b] - sd, page = pool + pages + b;
+    if (q * 8u < cnt)
+      __builtin_nontemporal_store(
+          ((const u32x4*)(ring + ((size_t)b << rsh) + ((size_t)((sd >> psh) & 1u) << psh)))[q],
+          (u32x4*)out + ((size_t)page << vsh) + q);
+    if (q == 0) {
+      tags[page] = b | (cnt << 16);
+      rows[(size_t)c * B + b] = (sd >> psh) + (cnt ? 1u : 0u);
+    }
+  }
+  if (HOT)
+    for (int k = tid; k < kHotSlots; k += nthr)
+      if (hcnt[k]) atomic_add_dev(counts + hkeys[k], (CT)hcnt[k]);
+  if (tid == 0) used[c] = pages + (uint32_t)B;
+  flush_errors(err, bad_r, bad_p);
+}
+
+// The sample's verdict (flags) picks the instantiation and the ring depth:
+// R with the hot table on, R_cold (deeper: the hot table's LDS is free) off.
+template <bool PACKED, bool HOT, bool RUNS, typename CT>
+__global__ __launch_bounds__(1024) void k_histo_scatter_ring(
+    const int64_t* __restrict__ idx, int64_t n, uint64_t nbins, int shift, int B, int R,
+    int R_cold, int64_t chunk, const uint32_t* __restrict__ hot_g, uint16_t* __restrict__ out,
+    uint32_t* __restrict__ tags, uint32_t* __restrict__ rows, uint32_t* __restrict__ used,
+    CT* __restrict__ counts, ErrWords* err, const uint32_t* __restrict__ flags) {
+  const uint32_t fl = flags ? *flags : (kFlagHotBins | kFlagRunsSeen);
+  const bool h = HOT && (fl & kFlagHotBins), r = RUNS && (fl & kFlagRunsSeen);
+  const int Re = (!h && HOT && R_cold > 0) ? R_cold : R;
+  const uint32_t pool = (uint32_t)(blockIdx.x * part_pool_pages(chunk, B, Re / 2));
+  if (h && r)
+    ring_scatter_body<PACKED, HOT, RUNS, CT>(blockIdx.x, idx, n, nbins, shift, B, Re, chunk, hot_g,
+                                             out, tags, pool, rows, used, counts, err);
+  else if (h)
+    ring_scatter_body<PACKED, HOT, false, CT>(blockIdx.x, idx, n, nbins, shift, B, Re, chunk, hot_g,
+                                              out, tags, pool, rows, used, counts, err);
+  else if (r)
+    ring_scatter_body<PACKED, false, RUNS, CT>(blockIdx.x, idx, n, nbins, shift, B, Re, chunk,
+                                               hot_g, out, tags, pool, rows, used, counts, err);
+  else
+    ring_scatter_body<PACKED, false, false, CT>(blockIdx.x, idx, n, nbins, shift, B, Re, chunk,
+                                                hot_g, out, tags, pool, rows, used, counts, err);
+}
+
+// One workgroup per chunk: its nonempty pages, grouped by bucket.
+// plist[start[b] + rows[c][b] + k] = k-th page of bucket b from chunk c (any
+// order within a chunk), with bit 31 set on a short page.
 template <bool HOT>
 __global__ __launch_bounds__(1024) void k_page_list(const uint32_t* __restrict__ tags,
                                                     const uint32_t* __restrict__ rows,
@@ -400,6 +745,7 @@ __global__ __launch_bounds__(1024) void k_page_list(const uint32_t* __restrict__
   const uint32_t np = used[c];
   for (uint32_t k = threadIdx.x; k < np; k += blockDim.x) {
     const uint32_t t = tags[pool + k];
+    if ((t >> 16) == 0) continue;  // a bucket with no entries left at the end
     const uint32_t short_page = (t >> 16) < (uint32_t)Se ? 0x80000000u : 0u;
     plist[atomicAdd(&cur[t & 0xffffu], 1u)] = (pool + k) | short_page;
   }
@@ -485,7 +831,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     for (uint32_t k = tid; k < words; k += nthr) sh[k] = 0;
     __syncthreads();
     const uint32_t nv = (g1 - g0) << vsh;
-    constexpr int Q = 2;  // vectors per thread in flight
+    constexpr int Q = 4;  // vectors per thread in flight
     for (uint32_t v0 = tid; v0 < nv; v0 += Q * nthr) {
       uint32_t ent[Q], cnt[Q];
       u32x4 x[Q];
@@ -520,13 +866,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     // a bucket with one segment is this workgroup's alone: plain adds, unless
     // a spill already went to its counters atomically
     const bool plain = nseg == 1 && !(PK && s_spilled);
-    for (uint32_t k = tid; k < width; k += nthr) {
-      uint32_t c = PK ? (sh[k >> 1] >> ((k & 1u) * 16u)) & 0xffffu : sh[k];
-      if (c) {
-        if (plain) dst[k] += (CT)c;
-        else atomic_add_dev(dst + k, (CT)c);
-      }
-    }
+    flush_tile<CT, PK>(sh, width, dst, plain, tid, nthr);
     __syncthreads();
   }
 }
@@ -738,18 +1078,20 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   if (B > kMaxBuckets) return (int)hipErrorInvalidValue;
   int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
   const size_t budget = (size_t)d->lds_per_block / (size_t)gpc;
-  int S = part_pick_depth((int)B, sizeof(uint16_t), budget, HOT);
+  // ring depth with the hot table's LDS taken, and without it (adaptive plan)
+  const int R = ring_pick((int)B, budget, HOT);
   const bool adaptive = HOT && c.hot == 2;
-  int S_cold = adaptive ? part_pick_depth((int)B, sizeof(uint16_t), budget, false) : 0;
-  const int s_lo = S_cold > 0 ? std::min(S, S_cold) : S;
-  if (s_lo < 8) return 1;
+  const int R_cold = adaptive ? ring_pick((int)B, budget, false) : 0;
+  if (R < 16 || (adaptive && R_cold < 16)) return 1;  // pages would be under 16 bytes
+  const int P = R / 2, P_cold = R_cold / 2;
+  const int p_lo = adaptive ? std::min(P, P_cold) : P;
   int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
   int64_t chunk = ceil_div(ceil_div(n, C), kChunkAlign) * kChunkAlign;
   C = ceil_div(n, chunk);
   // pool sizes for either depth: pages (tags, list) and entries (staging)
-  const int64_t pages = C * part_pool_pages(chunk, (int)B, s_lo);
-  const int64_t entries = std::max(C * part_pool_pages(chunk, (int)B, S) * S,
-                                   S_cold > 0 ? C * part_pool_pages(chunk, (int)B, S_cold) * S_cold : 0);
+  const int64_t pages = C * part_pool_pages(chunk, (int)B, p_lo);
+  const int64_t entries = std::max(C * part_pool_pages(chunk, (int)B, P) * P,
+                                   adaptive ? C * part_pool_pages(chunk, (int)B, P_cold) * P_cold : 0);
   if (entries >= ((int64_t)1 << 32) || pages >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
   const size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
   const size_t tag_b = ((size_t)pages * 4 + 255) & ~(size_t)255;
@@ -766,22 +1108,22 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   if (HOT)
     hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins,
                        L.hot, flags);
-  size_t lds = part_scatter_lds((int)B, S, sizeof(uint16_t), HOT);
-  if (S_cold > 0) lds = std::max(lds, part_scatter_lds((int)B, S_cold, sizeof(uint16_t), false));
-  allow_lds(k_histo_scatter_paged<PACKED, HOT, RUNS, CT>, lds);
-  hipLaunchKernelGGL((k_histo_scatter_paged<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024),
-                     lds, s, idx, n, (uint64_t)nbins, tb, (int)B, S, chunk, L.hot, staged, tags,
-                     L.chunk, used, counts, d->err, (const uint32_t*)flags, S_cold);
+  size_t lds = ring_scatter_lds((int)B, R, HOT);
+  if (adaptive) lds = std::max(lds, ring_scatter_lds((int)B, R_cold, false));
+  allow_lds(k_histo_scatter_ring<PACKED, HOT, RUNS, CT>, lds);
+  hipLaunchKernelGGL((k_histo_scatter_ring<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024),
+                     lds, s, idx, n, (uint64_t)nbins, tb, (int)B, R, R_cold, chunk, L.hot, staged,
+                     tags, L.chunk, used, counts, d->err, (const uint32_t*)flags);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk, (int)C, (int)B,
                      L.total);
   // segments as in the counted plan, in pages of the hot-on depth
   const int64_t seg = std::max<int64_t>({4096, ceil_div(n, (int64_t)d->cus * 2), (int64_t)4 << tb});
-  const uint32_t seg_pages = (uint32_t)ceil_div(seg, S);
+  const uint32_t seg_pages = (uint32_t)ceil_div(seg, P);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, seg_pages, L.start,
                      L.segstart, L.misc);
   hipLaunchKernelGGL(k_page_list<HOT>, dim3((unsigned)C), dim3(1024), (size_t)B * 4, s,
                      (const uint32_t*)tags, (const uint32_t*)L.chunk, (const uint32_t*)L.start,
-                     (const uint32_t*)used, (int)B, chunk, S, S_cold, (const uint32_t*)flags, plist);
+                     (const uint32_t*)used, (int)B, chunk, P, P_cold, (const uint32_t*)flags, plist);
   size_t lds_apply = (size_t)(pk ? 2 : 4) << tb;
   int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
   if (pk) {
@@ -789,14 +1131,14 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
     hipLaunchKernelGGL((k_histo_apply_paged<CT, HOT, RUNS, true>), dim3((unsigned)groups),
                        dim3(1024), lds_apply, s, (const uint16_t*)staged, (const uint32_t*)tags,
                        (const uint32_t*)plist, (const uint32_t*)L.start, (const uint32_t*)L.segstart,
-                       L.misc, (int)B, seg_pages, tb, (uint64_t)nbins, S, S_cold, counts,
+                       L.misc, (int)B, seg_pages, tb, (uint64_t)nbins, P, P_cold, counts,
                        (const uint32_t*)flags);
   } else {
     allow_lds(k_histo_apply_paged<CT, HOT, RUNS, false>, lds_apply);
     hipLaunchKernelGGL((k_histo_apply_paged<CT, HOT, RUNS, false>), dim3((unsigned)groups),
                        dim3(1024), lds_apply, s, (const uint16_t*)staged, (const uint32_t*)tags,
                        (const uint32_t*)plist, (const uint32_t*)L.start, (const uint32_t*)L.segstart,
-                       L.misc, (int)B, seg_pages, tb, (uint64_t)nbins, S, S_cold, counts,
+                       L.misc, (int)B, seg_pages, tb, (uint64_t)nbins, P, P_cold, counts,
                        (const uint32_t*)flags);
   }
   return (int)hipGetLastError();

@@ -12,8 +12,7 @@
 //   k_part_scan     prefix over buckets: bucket starts and the apply segment list
 //   part_scatter    one LDS send buffer per bucket; a full buffer is shipped with
 //                   one coalesced write to the chunk's own, exactly sized range of
-//                   the bucket (LDS cursor, no device atomics), or, paged, to the
-//                   next page of the chunk's own pool (no count pass needed)
+//                   the bucket (LDS cursor, no device atomics)
 //
 // Hot bins (histogram only): a sample of the input is sorted in LDS and bins
 // seen at least twice become "hot"; they go to an LDS hash table in every
@@ -361,10 +360,9 @@ __device__ inline void part_scan_body(const uint32_t* __restrict__ count, int B,
 }
 
 // ---- pass 3: scatter through LDS send buffers -------------------------------------
-// LDS layout: cursor[B], fill[B], list[B], nlist[4], sink[64], [hot keys, hot
-// counts], buf[B*S].
+// LDS layout: cursor[B], fill[B], list[B], nlist[2], [hot keys, hot counts], buf[B*S].
 inline size_t part_scatter_lds(int B, int S, size_t esize, bool hot) {
-  size_t head = (size_t)(3 * B + 4 + 64) * 4 + (hot ? (size_t)2 * kHotSlots * 4 : 0);
+  size_t head = (size_t)(3 * B + 2 + 64) * 4 + (hot ? (size_t)2 * kHotSlots * 4 : 0);
   return ((head + 15) & ~(size_t)15) + (size_t)B * S * esize;
 }
 // Largest power-of-two buffer depth S in [4, 256] that fits `budget` bytes (0 if none).
@@ -376,28 +374,6 @@ inline int part_pick_depth(int B, size_t esize, size_t budget, bool hot) {
 
 constexpr int kScatterU = 8;  // elements per thread per round (loaded a round ahead)
 
-// Where shipped buffers go.  Counted (the default): into the chunk's exact
-// range of each bucket, known from a count pass (rows, start).  Paged: into
-// the chunk's own pool of pages, one page per shipped buffer (S entries; the
-// last buffer of each bucket becomes a short page), in ship order, so no count
-// pass is needed.  tags[page] = bucket | (entries << 16); rows[c*B + b]
-// receives the number of bucket b's pages in chunk c and used[c] the pages
-// chunk c filled; index errors are counted into err (the counted plan's count
-// pass reports them).
-struct CountedShip {
-  static constexpr bool kPaged = false;
-};
-struct PagedShip {
-  static constexpr bool kPaged = true;
-  uint32_t* tags;
-  uint32_t pool;  // first page of this chunk's pool
-  uint32_t* rows;
-  uint32_t* used;
-  ErrWords* err;
-};
-// Pages a chunk of `len` updates can fill: full pages plus one short page per bucket.
-__host__ __device__ inline int64_t part_pool_pages(int64_t len, int B, int S) { return (len + S - 1) / S + B; }
-
 // MakeEntry(e, j, low) -> entry stored for element e with decoded index j
 // (low = j & ((1<<shift)-1)).  HotFlush(bin, count) adds count to a bin
 // directly (hot counters at the end, long runs as they are met).
@@ -407,31 +383,29 @@ __host__ __device__ inline int64_t part_pool_pages(int64_t len, int B, int S) { 
 // Store(pos, entry) writes one shipped entry to staging position pos;
 // Skip(e) is called for an element whose index is out of range.
 template <bool PACKED, bool HOT, bool RUNS, typename E, typename MakeEntry, typename HotFlush,
-          typename Store, typename Skip, typename Ship = CountedShip>
+          typename Store, typename Skip>
 __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ idx, int64_t n,
                                          uint64_t limit, int shift, int B, int S, int64_t chunk,
                                          const uint32_t* __restrict__ rows,
                                          const uint32_t* __restrict__ start,
                                          const uint32_t* __restrict__ hot_g, Store store,
-                                         MakeEntry make, HotFlush hot_flush, Skip skip,
-                                         Ship ship_to = Ship()) {
-  constexpr bool use_hot = HOT, use_runs = RUNS, paged = Ship::kPaged;
+                                         MakeEntry make, HotFlush hot_flush, Skip skip) {
+  constexpr bool use_hot = HOT, use_runs = RUNS;
   extern __shared__ __align__(16) unsigned char smem[];
-  // cursor: counted, the next staging slot of this chunk per bucket; paged,
-  // the pages shipped per bucket
-  uint32_t* cursor = (uint32_t*)smem;  // [B]
+  uint32_t* cursor = (uint32_t*)smem;  // [B] next staging slot of this chunk per bucket
   uint32_t* fill = cursor + B;         // [B]
   uint32_t* list = fill + B;           // [B] buckets to ship this step
-  uint32_t* nlist = list + B;          // [4] list lengths, alternating between steps; [2] short pages
-  uint32_t* sink = nlist + 4;          // [64] per-lane target of placements that place nothing
+  uint32_t* nlist = list + B;          // [2] list lengths, alternating between steps
+  uint32_t* sink = nlist + 2;          // [64] per-lane target of placements that place nothing
   uint32_t* hkeys = sink + 64;         // [kHotSlots]
   uint32_t* hcnt = hkeys + kHotSlots;  // [kHotSlots]
-  size_t head = (size_t)(3 * B + 4 + 64) * 4 + (HOT ? (size_t)2 * kHotSlots * 4 : 0);
+  size_t head = (size_t)(3 * B + 2 + 64) * 4 + (HOT ? (size_t)2 * kHotSlots * 4 : 0);
   E* buf = (E*)(smem + ((head + 15) & ~(size_t)15));  // [B*S]
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = __lane_id(), wave = tid >> 6, nwaves = nthr >> 6;
+  const uint32_t* row = rows + (size_t)c * B;
   for (int b = tid; b < B; b += nthr) {
-    cursor[b] = paged ? 0u : start[b] + rows[(size_t)c * B + b];
+    cursor[b] = start[b] + row[b];
     fill[b] = 0;
   }
   if (use_hot)
@@ -439,35 +413,25 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
       hkeys[k] = hot_g[k];
       hcnt[k] = 0;
     }
-  if (tid == 0) nlist[0] = nlist[1] = nlist[2] = 0;
+  if (tid == 0) nlist[0] = nlist[1] = 0;
   __syncthreads();
   const uint64_t mask = (1ULL << shift) - 1;
   int64_t lo = c * chunk;
   int64_t hi = lo + chunk < n ? lo + chunk : n;
   uint32_t p = 0;  // which list length this step appends to
-  uint32_t pages = 0;  // paged: full pages shipped so far (the same in every thread)
-  unsigned bad_r = 0, bad_p = 0;
 
   // Ship every full buffer listed this step: wave w takes list entries
   // w, w + nwaves, ...; its lanes reserve the entries' output ranges at once
   // (a bucket is listed at most once per step, so its cursor has one owner),
-  // then copy 64/S buffers per instruction.  Paged: list entry k goes to page
-  // `pages + k` of the chunk's pool.
+  // then copy 64/S buffers per instruction.
   auto ship_listed = [&](uint32_t nl) {
     for (uint32_t g0 = wave; g0 < nl; g0 += (uint32_t)nwaves * 64u) {
       uint32_t k = g0 + (uint32_t)lane * nwaves;
       uint32_t b_l = 0, pos_l = 0;
       if (k < nl) {
         b_l = list[k];
-        if constexpr (Ship::kPaged) {
-          const uint32_t page = ship_to.pool + pages + k;
-          pos_l = page * (uint32_t)S;
-          cursor[b_l] += 1u;
-          ship_to.tags[page] = b_l | ((uint32_t)S << 16);
-        } else {
-          pos_l = cursor[b_l];
-          cursor[b_l] = pos_l + (uint32_t)S;
-        }
+        pos_l = cursor[b_l];
+        cursor[b_l] = pos_l + (uint32_t)S;
         fill[b_l] -= (uint32_t)S;  // entries past S (overflow) stay counted
       }
       const uint32_t cnt = min(64u, (nl - g0 + nwaves - 1) / (uint32_t)nwaves);
@@ -489,19 +453,9 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
   };
 
   // one wave ships the first `cnt` entries of bucket b to the chunk's range
-  // (paged: to a short page after the chunk's full pages)
   auto ship = [&](int b, uint32_t cnt) {
     uint32_t pos = 0;
-    if (lane == 0) {
-      if constexpr (Ship::kPaged) {
-        const uint32_t page = ship_to.pool + pages + atomicAdd(&nlist[2], 1u);
-        cursor[b] += 1u;
-        ship_to.tags[page] = (uint32_t)b | (cnt << 16);
-        pos = page * (uint32_t)S;
-      } else {
-        pos = atomicAdd(&cursor[b], cnt);  // LDS atomic
-      }
-    }
+    if (lane == 0) pos = atomicAdd(&cursor[b], cnt);  // LDS atomic
     pos = __shfl(pos, 0);
     for (uint32_t k = lane; k < cnt; k += 64) store(pos + k, buf[(size_t)b * S + k]);
   };
@@ -534,7 +488,8 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
       uint64_t j = 0;
       bool ok = false;
       if (e < hi) {
-        ok = decode_index<PACKED>(raw[u], limit, j, bad_r, bad_p);
+        unsigned br = 0, bp = 0;
+        ok = decode_index<PACKED>(raw[u], limit, j, br, bp);
         if (ok && use_hot) hs = hot_find(hkeys, (uint32_t)j);
         if (!ok) skip(e);
       }
@@ -593,9 +548,7 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
       // ships take S entries off each listed bucket; the other list length
       // (last used a round ago, whose readers are past the barrier above) is
       // cleared here
-      const uint32_t nl = nlist[p];
-      ship_listed(nl);
-      pages += nl;
+      ship_listed(nlist[p]);
       if (tid == 0) nlist[p ^ 1] = 0;
       p ^= 1;
       // ships have read their buffers before overflow entries (and the next
@@ -626,42 +579,35 @@ __device__ inline void part_scatter_body(int64_t c, const int64_t* __restrict__ 
   if (use_hot)
     for (int k = tid; k < kHotSlots; k += nthr)
       if (hcnt[k]) hot_flush(hkeys[k], hcnt[k]);
-  if constexpr (Ship::kPaged) {
-    __syncthreads();
-    for (int b = tid; b < B; b += nthr) ship_to.rows[(size_t)c * B + b] = cursor[b];
-    if (tid == 0) ship_to.used[c] = pages + nlist[2];
-    flush_errors(ship_to.err, bad_r, bad_p);
-  }
 }
 
 // The scatter pass under the count pass's choice from `flags` (see
 // k_part_count).  Without the hot table, its LDS goes to deeper buffers:
 // S_cold entries per bucket (the caller sized the allocation for both).
 template <bool PACKED, bool HOT, bool RUNS, typename E, typename MakeEntry, typename HotFlush,
-          typename Store, typename Skip, typename Ship = CountedShip>
+          typename Store, typename Skip>
 __device__ inline void part_scatter_adaptive(int64_t c, const int64_t* __restrict__ idx, int64_t n,
                                              uint64_t limit, int shift, int B, int S, int S_cold,
                                              int64_t chunk, const uint32_t* __restrict__ rows,
                                              const uint32_t* __restrict__ start,
                                              const uint32_t* __restrict__ hot_g, Store store,
                                              MakeEntry make, HotFlush hot_flush, Skip skip,
-                                             const uint32_t* __restrict__ flags,
-                                             Ship ship_to = Ship()) {
+                                             const uint32_t* __restrict__ flags) {
   const uint32_t fl = flags ? *flags : (kFlagHotBins | kFlagRunsSeen);
   const bool h = HOT && (fl & kFlagHotBins), r = RUNS && (fl & kFlagRunsSeen);
   const int Sc = S_cold > 0 ? S_cold : S;
   if (h && r)
     part_scatter_body<PACKED, HOT, RUNS, E>(c, idx, n, limit, shift, B, S, chunk, rows, start,
-                                            hot_g, store, make, hot_flush, skip, ship_to);
+                                            hot_g, store, make, hot_flush, skip);
   else if (h)
     part_scatter_body<PACKED, HOT, false, E>(c, idx, n, limit, shift, B, S, chunk, rows, start,
-                                             hot_g, store, make, hot_flush, skip, ship_to);
+                                             hot_g, store, make, hot_flush, skip);
   else if (r)
     part_scatter_body<PACKED, false, RUNS, E>(c, idx, n, limit, shift, B, HOT ? Sc : S, chunk,
-                                              rows, start, hot_g, store, make, hot_flush, skip, ship_to);
+                                              rows, start, hot_g, store, make, hot_flush, skip);
   else
     part_scatter_body<PACKED, false, false, E>(c, idx, n, limit, shift, B, HOT ? Sc : S, chunk,
-                                               rows, start, hot_g, store, make, hot_flush, skip, ship_to);
+                                               rows, start, hot_g, store, make, hot_flush, skip);
 }
 
 }  // namespace bale
