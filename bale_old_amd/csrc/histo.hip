@@ -226,22 +226,71 @@ __device__ inline void lds_add8(uint32_t* sh, u32x4 x) {
 // atomic returning and that lane's correction: thousands of cycles of a
 // wave that is ready to issue.
 constexpr uint32_t kSpill = 1024;
+// The kSpill moved off an LDS word is counted in LDS too: a bit per bin says
+// the bin spilled, a small hash table (bin -> number of spills) counts them,
+// and the flush adds kSpill per spill.  A segment of m updates spills at most
+// m / kSpill times, so the table (kSpillSlots) fills only for segments of more
+// than kSpillSlots * kSpill updates; then the spill goes to the device counter
+// with an atomic, and `flag` makes the flush atomic as well.
+constexpr int kSpillBits = 10;
+constexpr uint32_t kSpillSlots = 1u << kSpillBits;
+struct PkSpill {
+  uint32_t* flag;  // some spill went to a device counter
+  uint32_t* bits;  // [2^tb / 32]
+  uint32_t* keys;  // [kSpillSlots], kHotEmpty when free
+  uint32_t* cnts;  // [kSpillSlots]
+};
+__device__ inline uint32_t spill_hash(uint32_t bin) { return (bin * 0x9E3779B1u) >> (32 - kSpillBits); }
+__device__ inline bool pk_spill_note(const PkSpill& sp, uint32_t bin) {
+  uint32_t h = spill_hash(bin);
+  for (uint32_t probe = 0; probe < kSpillSlots; probe++) {
+    const uint32_t prev = atomicCAS(&sp.keys[h], kHotEmpty, bin);
+    if (prev == kHotEmpty || prev == bin) {
+      atomicAdd(&sp.cnts[h], 1u);
+      atomicOr(&sp.bits[bin >> 5], 1u << (bin & 31u));
+      return true;
+    }
+    h = (h + 1u) & (kSpillSlots - 1u);
+  }
+  return false;
+}
+// Spills recorded for a bin whose bit is set (after the segment's barrier).
+__device__ inline uint32_t pk_spill_count(const PkSpill& sp, uint32_t bin) {
+  uint32_t h = spill_hash(bin);
+  for (uint32_t probe = 0; probe < kSpillSlots; probe++) {
+    if (sp.keys[h] == bin) return sp.cnts[h];
+    h = (h + 1u) & (kSpillSlots - 1u);
+  }
+  return 0;
+}
+// LDS words after a packed tile of 2^tb bins: spill bits and table.
+__host__ __device__ inline uint32_t pk_spill_words(int tb) { return (1u << tb) / 32u + 2u * kSpillSlots; }
 
 template <typename CT>
 __device__ inline void pk_fix(uint32_t* sh, uint32_t bin, uint32_t inc, uint32_t old, CT* dst,
-                              uint32_t* spilled) {
+                              const PkSpill& sp) {
   const uint32_t sft = (bin & 1u) * 16u;
   if ((((old >> sft) & 0xffffu) & (kSpill - 1u)) + inc >= kSpill) {
     atomicAdd(&sh[bin >> 1], 0u - (kSpill << sft));
-    atomic_add_dev(dst + bin, (CT)kSpill);
-    *spilled = 1u;
+    if (!pk_spill_note(sp, bin)) {
+      atomic_add_dev(dst + bin, (CT)kSpill);
+      *sp.flag = 1u;
+    }
+  }
+}
+// Zero a packed tile's spill bits and table.
+__device__ inline void pk_spill_clear(const PkSpill& sp, int tb, int tid, int nthr) {
+  for (uint32_t k = tid; k < (1u << tb) / 32u; k += nthr) sp.bits[k] = 0;
+  for (uint32_t k = tid; k < kSpillSlots; k += nthr) {
+    sp.keys[k] = kHotEmpty;
+    sp.cnts[k] = 0;
   }
 }
 
 // lds_add8 for the packed tile; all of a call's atomics are issued before
 // their results are checked.
 template <bool RUNS, typename CT>
-__device__ inline void lds_add8_pk(uint32_t* sh, u32x4 x, CT* dst, uint32_t* spilled) {
+__device__ inline void lds_add8_pk(uint32_t* sh, u32x4 x, CT* dst, const PkSpill& spilled) {
   uint32_t v[8] = {x.x & 0xffff, x.x >> 16, x.y & 0xffff, x.y >> 16,
                    x.z & 0xffff, x.z >> 16, x.w & 0xffff, x.w >> 16};
   uint32_t inc[8], old[8];
@@ -265,7 +314,7 @@ __device__ inline void lds_add8_pk(uint32_t* sh, u32x4 x, CT* dst, uint32_t* spi
 }
 
 template <typename CT>
-__device__ inline void lds_add1_pk(uint32_t* sh, uint32_t bin, CT* dst, uint32_t* spilled) {
+__device__ inline void lds_add1_pk(uint32_t* sh, uint32_t bin, CT* dst, const PkSpill& spilled) {
   const uint32_t old = atomicAdd(&sh[bin >> 1], 1u << ((bin & 1u) * 16u));
   pk_fix(sh, bin, 1u, old, dst, spilled);
 }
@@ -277,8 +326,11 @@ __device__ inline void lds_add1_pk(uint32_t* sh, uint32_t bin, CT* dst, uint32_t
 // streams instead of waiting on one load at a time.
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 template <bool PK>
-__device__ inline uint32_t tile_count(const uint32_t* sh, uint32_t k) {
-  return PK ? (sh[k >> 1] >> ((k & 1u) * 16u)) & 0xffffu : sh[k];
+__device__ inline uint32_t tile_count(const uint32_t* sh, uint32_t k, const PkSpill& sp) {
+  if (!PK) return sh[k];
+  uint32_t c = (sh[k >> 1] >> ((k & 1u) * 16u)) & 0xffffu;
+  if ((sp.bits[k >> 5] >> (k & 31u)) & 1u) c += kSpill * pk_spill_count(sp, k);
+  return c;
 }
 // four consecutive counters as 16-byte vectors
 struct Cnt4_64 { i64x2 a, b; };
@@ -295,10 +347,10 @@ __device__ inline void store4(int32_t* p, i32x4 v, const uint32_t* c) {
 }
 template <typename CT, bool PK>
 __device__ inline void flush_tile(const uint32_t* sh, uint32_t width, CT* dst, bool plain, int tid,
-                                  int nthr) {
+                                  int nthr, const PkSpill& sp) {
   if (!plain) {
     for (uint32_t k = tid; k < width; k += nthr) {
-      const uint32_t c = tile_count<PK>(sh, k);
+      const uint32_t c = tile_count<PK>(sh, k, sp);
       if (c) atomic_add_dev(dst + k, (CT)c);
     }
     return;
@@ -314,7 +366,7 @@ __device__ inline void flush_tile(const uint32_t* sh, uint32_t width, CT* dst, b
     for (int f = 0; f < F; f++) {
       const uint32_t k = g0 + f * step;
 #pragma unroll
-      for (int i = 0; i < 4; i++) c[f][i] = k < w4 ? tile_count<PK>(sh, k + i) : 0u;
+      for (int i = 0; i < 4; i++) c[f][i] = k < w4 ? tile_count<PK>(sh, k + i, sp) : 0u;
       any[f] = (c[f][0] | c[f][1] | c[f][2] | c[f][3]) != 0;
     }
     // all loads first, then the stores
@@ -326,7 +378,7 @@ __device__ inline void flush_tile(const uint32_t* sh, uint32_t width, CT* dst, b
       if (any[f]) store4(dst + g0 + f * step, v[f], c[f]);
   }
   for (uint32_t k = w4 + tid; k < width; k += nthr) {
-    const uint32_t c = tile_count<PK>(sh, k);
+    const uint32_t c = tile_count<PK>(sh, k, sp);
     if (c) dst[k] += (CT)c;
   }
 }
@@ -345,7 +397,11 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
                                                       const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
   __shared__ uint32_t s_work[4];
-  __shared__ uint32_t s_spilled;  // PK: some counter of this segment spilled
+  __shared__ uint32_t s_spilled;  // PK: some spill of this segment went to a device counter
+  // PK: spill bits and table after the tile (2^(shift-1) words)
+  uint32_t* const spill_base = sh + (PK ? 1u << (shift - 1) : 0u);
+  const PkSpill sp{&s_spilled, spill_base, spill_base + (1u << shift) / 32u,
+                   spill_base + (1u << shift) / 32u + kSpillSlots};
   const bool merge = RUNS && (!flags || (*flags & kFlagRunsSeen));
   const int tid = threadIdx.x, nthr = blockDim.x;
   const uint32_t total = misc[0];
@@ -378,11 +434,12 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
     const uint32_t words = PK ? (width + 1u) >> 1 : width;
     CT* dst = counts + lo_bin;
     for (uint32_t k = tid; k < words; k += nthr) sh[k] = 0;
+    if (PK) pk_spill_clear(sp, shift, tid, nthr);
     __syncthreads();
     // unaligned head, then 8 entries per 16-byte load
     uint32_t a0 = min(e1, (e0 + 7u) & ~7u);
     for (uint32_t k = e0 + tid; k < a0; k += nthr) {
-      if (PK) lds_add1_pk(sh, staged[k], dst, &s_spilled);
+      if (PK) lds_add1_pk(sh, staged[k], dst, sp);
       else atomicAdd(&sh[staged[k]], 1u);
     }
     uint32_t nvec = (e1 - a0) / 8u;
@@ -390,22 +447,22 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
     for (uint32_t q = tid; q < nvec; q += nthr) {
       u32x4 x = __builtin_nontemporal_load(v + q);
       if (PK) {
-        if (merge) lds_add8_pk<true>(sh, x, dst, &s_spilled);
-        else lds_add8_pk<false>(sh, x, dst, &s_spilled);
+        if (merge) lds_add8_pk<true>(sh, x, dst, sp);
+        else lds_add8_pk<false>(sh, x, dst, sp);
       } else {
         if (merge) lds_add8<true>(sh, x);
         else lds_add8<false>(sh, x);
       }
     }
     for (uint32_t k = a0 + nvec * 8u + tid; k < e1; k += nthr) {
-      if (PK) lds_add1_pk(sh, staged[k], dst, &s_spilled);
+      if (PK) lds_add1_pk(sh, staged[k], dst, sp);
       else atomicAdd(&sh[staged[k]], 1u);
     }
     __syncthreads();
     // a bucket with one segment is this workgroup's alone: plain adds, unless
     // a spill already went to its counters atomically
     const bool plain = nseg == 1 && !(PK && s_spilled);
-    flush_tile<CT, PK>(sh, width, dst, plain, tid, nthr);
+    flush_tile<CT, PK>(sh, width, dst, plain, tid, nthr, sp);
     __syncthreads();
   }
 }
@@ -726,8 +783,16 @@ __global__ __launch_bounds__(1024) void k_histo_scatter_ring(
 }
 
 // One workgroup per chunk: its nonempty pages, grouped by bucket.
-// plist[start[b] + rows[c][b] + k] = k-th page of bucket b from chunk c (any
-// order within a chunk), with bit 31 set on a short page.
+// plist[start[b] + rows[c][b] + k] = k-th page of bucket b from chunk c, with
+// bit 31 set on a short page.  Pages go through LDS a window at a time,
+// counting-sorted by bucket, so each bucket's pages of a window are written as
+// one contiguous run.
+// LDS: cur[B] (next list position per bucket), cnt[B], lstart[B],
+// wsum[32], sorted[kListWin], sbkt[kListWin] (u16).
+constexpr int kListWin = 8192;
+__host__ __device__ inline size_t page_list_lds(int B) {
+  return (size_t)(3 * B + 32 + kListWin) * 4 + (size_t)kListWin * 2;
+}
 template <bool HOT>
 __global__ __launch_bounds__(1024) void k_page_list(const uint32_t* __restrict__ tags,
                                                     const uint32_t* __restrict__ rows,
@@ -736,25 +801,85 @@ __global__ __launch_bounds__(1024) void k_page_list(const uint32_t* __restrict__
                                                     int64_t chunk, int S, int S_cold,
                                                     const uint32_t* __restrict__ flags,
                                                     uint32_t* __restrict__ plist) {
-  extern __shared__ uint32_t cur[];
-  const int c = blockIdx.x;
+  extern __shared__ uint32_t lds[];
+  uint32_t* cur = lds;
+  uint32_t* cnt = cur + B;
+  uint32_t* lstart = cnt + B;
+  uint32_t* wsum = lstart + B;
+  uint32_t* sorted = wsum + 32;
+  uint16_t* sbkt = (uint16_t*)(sorted + kListWin);
+  constexpr int PER = kListWin / 1024;  // pages per thread per window
+  const int c = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
   const int Se = paged_depth<HOT>(flags, S, S_cold);
   const uint32_t pool = (uint32_t)(c * part_pool_pages(chunk, B, Se));
-  for (int b = threadIdx.x; b < B; b += blockDim.x) cur[b] = start[b] + rows[(size_t)c * B + b];
-  __syncthreads();
+  for (int b = tid; b < B; b += 1024) cur[b] = start[b] + rows[(size_t)c * B + b];
   const uint32_t np = used[c];
-  for (uint32_t k = threadIdx.x; k < np; k += blockDim.x) {
-    const uint32_t t = tags[pool + k];
-    if ((t >> 16) == 0) continue;  // a bucket with no entries left at the end
-    const uint32_t short_page = (t >> 16) < (uint32_t)Se ? 0x80000000u : 0u;
-    plist[atomicAdd(&cur[t & 0xffffu], 1u)] = (pool + k) | short_page;
+  const int per_b = (B + 1023) / 1024;  // buckets per thread in the scan
+  for (uint32_t w0 = 0; w0 < np; w0 += kListWin) {
+    for (int b = tid; b < B; b += 1024) cnt[b] = 0;
+    __syncthreads();
+    // count and rank this window's pages per bucket
+    uint32_t bk[PER], rk[PER], pg[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const uint32_t k = w0 + q * 1024 + tid;
+      const uint32_t t = k < np ? tags[pool + k] : 0u;
+      bk[q] = t & 0xffffu;
+      pg[q] = (pool + k) | ((t >> 16) < (uint32_t)Se ? 0x80000000u : 0u);
+      rk[q] = (t >> 16) ? atomicAdd(&cnt[bk[q]], 1u) : ~0u;  // ~0: empty or past the end
+    }
+    __syncthreads();
+    // exclusive scan of cnt into lstart: per-thread sums of per_b buckets,
+    // then across the workgroup (waves, then the 16 wave totals)
+    uint32_t mine = 0;
+    for (int i = 0; i < per_b; i++) {
+      const int b = tid * per_b + i;
+      if (b < B) mine += cnt[b];
+    }
+    uint32_t incl = mine;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < wave; w++) before += wsum[w];
+    uint32_t run = before + incl - mine;
+    for (int i = 0; i < per_b; i++) {
+      const int b = tid * per_b + i;
+      if (b < B) {
+        lstart[b] = run;
+        run += cnt[b];
+      }
+    }
+    __syncthreads();
+    // place into the sorted window
+#pragma unroll
+    for (int q = 0; q < PER; q++)
+      if (rk[q] != ~0u) {
+        const uint32_t pos = lstart[bk[q]] + rk[q];
+        sorted[pos] = pg[q];
+        sbkt[pos] = (uint16_t)bk[q];
+      }
+    __syncthreads();
+    // write: element i of the sorted window is page i - lstart[b] of bucket b here
+    uint32_t total = 0;
+    for (int w = 0; w < 16; w++) total += wsum[w];
+    for (uint32_t i = tid; i < total; i += 1024) {
+      const uint32_t b = sbkt[i];
+      plist[cur[b] + (i - lstart[b])] = sorted[i];
+    }
+    __syncthreads();
+    for (int b = tid; b < B; b += 1024) cur[b] += cnt[b];
+    // the next window's first barrier orders these updates before their reads
   }
 }
 
 // lds_add8 / lds_add8_pk for the first `valid` (1..8) of the 8 offsets in x.
 template <bool RUNS, bool PK, typename CT>
 __device__ inline void lds_add_valid(uint32_t* sh, u32x4 x, uint32_t valid, CT* dst,
-                                     uint32_t* spilled) {
+                                     const PkSpill& spilled) {
   uint32_t v[8] = {x.x & 0xffff, x.x >> 16, x.y & 0xffff, x.y >> 16,
                    x.z & 0xffff, x.z >> 16, x.w & 0xffff, x.w >> 16};
   uint32_t inc[8], old[8];
@@ -795,7 +920,11 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
   __shared__ uint32_t s_work[2];
-  __shared__ uint32_t s_spilled;  // PK: some counter of this segment spilled
+  __shared__ uint32_t s_spilled;  // PK: some spill of this segment went to a device counter
+  // PK: spill bits and table after the tile (2^(shift-1) words)
+  uint32_t* const spill_base = sh + (PK ? 1u << (shift - 1) : 0u);
+  const PkSpill sp{&s_spilled, spill_base, spill_base + (1u << shift) / 32u,
+                   spill_base + (1u << shift) / 32u + kSpillSlots};
   const bool merge = RUNS && (!flags || (*flags & kFlagRunsSeen));
   const uint32_t Se = (uint32_t)paged_depth<HOT>(flags, S, S_cold);
   const uint32_t vpp = Se / 8u;  // 16-byte vectors per page (a power of two)
@@ -829,6 +958,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     const uint32_t words = PK ? (width + 1u) >> 1 : width;
     CT* dst = counts + lo_bin;
     for (uint32_t k = tid; k < words; k += nthr) sh[k] = 0;
+    if (PK) pk_spill_clear(sp, shift, tid, nthr);
     __syncthreads();
     const uint32_t nv = (g1 - g0) << vsh;
     constexpr int Q = 4;  // vectors per thread in flight
@@ -857,8 +987,8 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
         const uint32_t first = (v & (vpp - 1u)) * 8u;
         if (v < nv && cnt[q] > first) {
           const uint32_t valid = min(8u, cnt[q] - first);
-          if (merge) lds_add_valid<true, PK>(sh, x[q], valid, dst, &s_spilled);
-          else lds_add_valid<false, PK>(sh, x[q], valid, dst, &s_spilled);
+          if (merge) lds_add_valid<true, PK>(sh, x[q], valid, dst, sp);
+          else lds_add_valid<false, PK>(sh, x[q], valid, dst, sp);
         }
       }
     }
@@ -866,7 +996,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     // a bucket with one segment is this workgroup's alone: plain adds, unless
     // a spill already went to its counters atomically
     const bool plain = nseg == 1 && !(PK && s_spilled);
-    flush_tile<CT, PK>(sh, width, dst, plain, tid, nthr);
+    flush_tile<CT, PK>(sh, width, dst, plain, tid, nthr, sp);
     __syncthreads();
   }
 }
@@ -993,6 +1123,14 @@ int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins,
 
 constexpr int kMaxBuckets = 4096;
 
+// Updates per apply segment: long enough that a segment's flush (up to one add
+// per bin of the tile) stays a fraction of its updates, short enough to spread
+// the pass over the chip; cfg.task_log2 sets it explicitly.
+static int64_t apply_segment(int64_t n, int tb, const bale_config& c, const Device* d) {
+  if (c.task_log2 > 0) return std::max<int64_t>(4096, (int64_t)1 << std::min(30, (int)c.task_log2));
+  return std::max<int64_t>({4096, ceil_div(n, (int64_t)d->cus * 2), (int64_t)4 << tb});
+}
+
 template <bool PACKED, bool HOT, bool RUNS, typename CT>
 static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins,
                         const bale_config& c, Device* d, hipStream_t s) {
@@ -1030,10 +1168,7 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
                      (const uint32_t*)flags);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);
-  // segments: enough of them to spread the apply pass over the chip
-  // Long enough that a segment's flush (up to one add per bin of the tile) stays
-  // a fraction of its updates, short enough to spread the pass over the chip.
-  int64_t seg = std::max<int64_t>({4096, ceil_div(n, (int64_t)d->cus * 2), (int64_t)4 << tb});
+  int64_t seg = apply_segment(n, tb, c, d);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
                      L.start, L.segstart, L.misc);
   size_t lds = part_scatter_lds((int)B, S, sizeof(uint16_t), HOT);
@@ -1048,7 +1183,7 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   hipLaunchKernelGGL((k_histo_scatter<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024), lds, s, idx,
                      n, (uint64_t)nbins, tb, (int)B, S, chunk, L.chunk, L.start, L.hot,
                      (uint16_t*)L.staged, counts, (const uint32_t*)flags, S_cold);
-  size_t lds_apply = (size_t)(pk ? 2 : 4) << tb;
+  size_t lds_apply = ((size_t)(pk ? 2 : 4) << tb) + (pk ? (size_t)pk_spill_words(tb) * 4 : 0);
   int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
   if (pk) {
     allow_lds(k_histo_apply<CT, RUNS, true>, lds_apply);
@@ -1116,15 +1251,16 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
                      tags, L.chunk, used, counts, d->err, (const uint32_t*)flags);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk, (int)C, (int)B,
                      L.total);
-  // segments as in the counted plan, in pages of the hot-on depth
-  const int64_t seg = std::max<int64_t>({4096, ceil_div(n, (int64_t)d->cus * 2), (int64_t)4 << tb});
+  // apply segments in pages of the hot-on depth
+  const int64_t seg = apply_segment(n, tb, c, d);
   const uint32_t seg_pages = (uint32_t)ceil_div(seg, P);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, seg_pages, L.start,
                      L.segstart, L.misc);
-  hipLaunchKernelGGL(k_page_list<HOT>, dim3((unsigned)C), dim3(1024), (size_t)B * 4, s,
+  allow_lds(k_page_list<HOT>, page_list_lds((int)B));
+  hipLaunchKernelGGL(k_page_list<HOT>, dim3((unsigned)C), dim3(1024), page_list_lds((int)B), s,
                      (const uint32_t*)tags, (const uint32_t*)L.chunk, (const uint32_t*)L.start,
                      (const uint32_t*)used, (int)B, chunk, P, P_cold, (const uint32_t*)flags, plist);
-  size_t lds_apply = (size_t)(pk ? 2 : 4) << tb;
+  size_t lds_apply = ((size_t)(pk ? 2 : 4) << tb) + (pk ? (size_t)pk_spill_words(tb) * 4 : 0);
   int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
   if (pk) {
     allow_lds(k_histo_apply_paged<CT, HOT, RUNS, true>, lds_apply);

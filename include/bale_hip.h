@@ -111,7 +111,8 @@ typedef struct bale_config {
                          sees twice in a per-workgroup LDS hash table; 0 = off;
                          BUCKET only: 2 = let that sample decide on the device, per
                          call, whether to use the hot table and run merging */
-  int32_t task_log2;  /* batches: operations per task (log2, 12..22); 0 = 16 */
+  int32_t task_log2;  /* batches: operations per task (log2, 12..22); 0 = 16.
+                         BUCKET histograms: updates per apply segment (log2); 0 = by size */
 } bale_config;
 
 /* Select the device for this thread's later calls and create its state.

@@ -146,6 +146,23 @@ def test_histo_bucket_pages(dev, orc, nbins, n, tile_log2, dist):
         assert np.array_equal(to_np(counts), want), (hot, merge)
 
 
+@pytest.mark.parametrize("distinct", [600, 1100])
+@pytest.mark.parametrize("dtype", [torch.int64, torch.int32])
+def test_histo_packed_tile_spill_table(dev, distinct, dtype):
+    """Spills of the packed 16-bit tile are counted in an LDS table of 1024
+    bins; one 2^21-update segment with 1100 bins that each spill overflows it,
+    and the rest go to device atomics (the flush then turns atomic too)."""
+    nbins, reps = 1 << 16, 1100
+    bins = (torch.arange(distinct, device=dev, dtype=torch.int64) * 59) % nbins
+    idx = bins.repeat(reps)
+    idx = idx[torch.randperm(idx.numel(), device=dev)]
+    want = np.bincount(to_np(idx), minlength=nbins) + 3
+    counts = torch.full((nbins,), 3, dtype=dtype, device=dev)
+    cfg = B.config(B.BUCKET, block=1024, tile_log2=16, hot=0, task_log2=21)
+    assert B.histo(idx, counts, cfg=cfg) == 0
+    assert np.array_equal(to_np(counts).astype(np.int64), want)
+
+
 def test_histo_packed_and_errors(dev, orc):
     nbins, n = 4096, 100_000
     idx = gen(dev, n, nbins, seed=9)
