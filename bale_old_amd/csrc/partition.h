@@ -34,18 +34,18 @@ constexpr int kSketch = 4096;            // sampled indices for the hot set
 constexpr uint32_t kLongRun = 4;
 constexpr int64_t kChunkAlign = 4096;
 
-__device__ inline uint32_t hot_hash(uint32_t key) { return (key * 0x9E3779B1u) >> (32 - 11); }
+// Hot-bin table: kHotSlots slots in aligned pairs.  A bin lives in one of the
+// two slots of the pair its hash names, or is left out of the hot set (it is
+// then counted like any other bin), so a lookup is two adjacent LDS reads and
+// no loop.
+__device__ inline uint32_t hot_pair(uint32_t key) { return ((key * 0x9E3779B1u) >> (32 - 10)) << 1; }
+static_assert(kHotSlots == 2 << 10, "hot_pair names kHotSlots / 2 pairs");
 
-// Slot of `key` in the hash table `keys`, or -1.
+// Slot of `key` in the hot table `keys`, or -1.
 __device__ inline int hot_find(const uint32_t* keys, uint32_t key) {
-  uint32_t h = hot_hash(key);
-  for (int probe = 0; probe < kHotSlots; probe++) {
-    uint32_t k = keys[h];
-    if (k == key) return (int)h;
-    if (k == kHotEmpty) return -1;
-    h = (h + 1) & (kHotSlots - 1);
-  }
-  return -1;
+  const uint32_t h = hot_pair(key);
+  const uint32_t k0 = keys[h], k1 = keys[h + 1];
+  return k0 == key ? (int)h : k1 == key ? (int)(h + 1) : -1;
 }
 
 // Workspace layout (32-bit words, then the staging array).  misc: [0] total
@@ -159,8 +159,9 @@ __device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t 
     if (c >= 2) {
       dup += c;
       if (atomicAdd(&nhot, 1u) < (uint32_t)kHotMax) {
-        uint32_t v = keys[h], t = hot_hash(v);
-        while (atomicCAS(&table[t], kHotEmpty, v) != kHotEmpty) t = (t + 1) & (kHotSlots - 1);
+        // first free slot of the bin's pair; a full pair leaves the bin cold
+        const uint32_t v = keys[h], t = hot_pair(v);
+        if (atomicCAS(&table[t], kHotEmpty, v) != kHotEmpty) (void)atomicCAS(&table[t + 1], kHotEmpty, v);
       }
     }
   }
