@@ -569,28 +569,29 @@ __device__ __attribute__((always_inline)) inline void ring_scatter_body(
     return valid && pe_ok && in;
   };
   // Count a hot bin in LDS or a long run with one device atomic; otherwise the
-  // update is placed (returns true) with its bucket and 16-bit entry.  Every
-  // lane of the wave calls it together (wave_runs).
-  auto classify = [&](bool ok, uint32_t j, uint32_t& bk, uint32_t& ent) -> bool {
+  // update is placed (returns true) with its bucket and 16-bit entry.  RUNS:
+  // one run detection on the bin (adjacent lanes with equal bins): a hot run
+  // is counted at its head, a long cold run is one device atomic, and a short
+  // cold run reserves its slots at its head (rh, rl).  Every lane of the wave
+  // calls it together (wave_runs).
+  auto classify = [&](bool ok, uint32_t j, uint32_t& bk, uint32_t& ent, int& rh,
+                      uint32_t& rl) -> bool {
     int hs = -1;
     if (HOT && ok) hs = hot_find(hkeys, j);
-    if (HOT) {
-      if (RUNS) {
-        uint32_t len;
-        const int h0 = wave_runs(hs >= 0 ? (uint32_t)hs : kHotEmpty, len);
-        if (hs >= 0 && lane == h0) atomicAdd(&hcnt[hs], len);
-      } else if (hs >= 0) {
-        atomicAdd(&hcnt[hs], 1u);
-      }
-    }
     bool cd = ok && hs < 0;
+    rh = lane;
+    rl = 1;
     if (RUNS) {
-      uint32_t len;
-      const int h0 = wave_runs(cd ? j : kHotEmpty, len);
-      if (cd && len >= kLongRun) {
-        if (lane == h0) atomic_add_dev(counts + j, (CT)len);
+      rh = wave_runs(ok ? j : kHotEmpty, rl);
+      const bool head = lane == rh;
+      if (HOT && hs >= 0) {
+        if (head) atomicAdd(&hcnt[hs], rl);
+      } else if (cd && rl >= kLongRun) {
+        if (head) atomic_add_dev(counts + j, (CT)rl);
         cd = false;
       }
+    } else if (HOT && hs >= 0) {
+      atomicAdd(&hcnt[hs], 1u);
     }
     bk = cd ? j >> shift : 0u;
     ent = j & mask;
@@ -651,7 +652,7 @@ __device__ __attribute__((always_inline)) inline void ring_scatter_body(
   // FULL: the whole round is inside the chunk (no bounds tests).
   auto round = [&](auto full_c, int64_t base, const i64x2* raw) {
     constexpr bool FULL = decltype(full_c)::value;
-    uint32_t bk[U], ent[U], s[U];
+    uint32_t bk[U], ent[U], s[U], rlen[U];
     int rhead[U];
     uint32_t cold = 0;
 #pragma unroll
@@ -662,17 +663,14 @@ __device__ __attribute__((always_inline)) inline void ring_scatter_body(
         const bool valid = FULL || base + 2 * (k * 1024 + tid) + h < hi;
         uint32_t j;
         const bool ok = decode(h ? raw[k].y : raw[k].x, valid, j);
-        cold |= (classify(ok, j, bk[u], ent[u]) ? 1u : 0u) << u;
+        cold |= (classify(ok, j, bk[u], ent[u], rhead[u], rlen[u]) ? 1u : 0u) << u;
       }
-    // reserve slots: one LDS atomic per update (per run of a bucket with RUNS);
-    // lanes with nothing to place add to their own sink word
+    // reserve slots: one LDS atomic per update (per run with RUNS); lanes with
+    // nothing to place add to their own sink word
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const bool cd = (cold >> u) & 1u;
-      uint32_t len = 1;
-      rhead[u] = lane;
-      if (RUNS) rhead[u] = wave_runs(cd ? bk[u] : kHotEmpty, len);
-      s[u] = atomicAdd((cd && lane == rhead[u]) ? &fill[bk[u]] : &sink[lane], len);
+      s[u] = atomicAdd((cd && lane == rhead[u]) ? &fill[bk[u]] : &sink[lane], rlen[u]);
     }
     uint32_t se[U];
 #pragma unroll
