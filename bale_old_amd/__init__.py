@@ -111,6 +111,13 @@ def _proto(L) -> None:
     f("bale_batch_launch_plain", C.c_int, IP, I64T, VP)
     f("bale_tune", C.c_int, IP, I64T, I64T, C.POINTER(C.c_float), C.c_int, VP)
     f("bale_tune_candidates", C.c_int, C.c_int, I64T, C.POINTER(BaleConfig), C.c_int)
+    f("bale_randperm", C.c_int, VP, I64T, I64T, VP)
+    f("bale_is_perm", C.c_int, VP, I64T, VP)
+    for name in ("rand_permp_agi", "rand_permp_conveyor"):
+        f(name, VP, I64T, C.c_int)
+    for name in ("rand_permp_exstack", "rand_permp_exstack2"):
+        f(name, VP, I64T, C.c_int, I64T)
+    f("is_perm", C.c_int, VP, I64T)
     f("bale_all_alloc", VP, I64T, I64T)
     f("bale_all_free", None, VP)
     f("bale_memput", C.c_int, VP, VP, I64T)
@@ -238,3 +245,17 @@ def last_timing() -> tuple[float, float, float]:
     g, h, d = C.c_double(), C.c_double(), C.c_double()
     lib().bale_hip_last_timing(C.byref(g), C.byref(h), C.byref(d))
     return g.value, h.value, d.value
+
+
+def randperm(perm, seed: int, stream=None) -> None:
+    """perm (int64 device tensor of length n) = the dart-throwing random
+    permutation of 0..n-1 (rand_permp_agi at one PE, spmat/spmat_agi.upc:56)."""
+    check(lib().bale_randperm(_ptr(perm), perm.numel(), seed, _stream(stream)))
+
+
+def is_perm(perm, stream=None) -> bool:
+    """Whether perm (int64 device tensor) holds 0..n-1 once each (on the GPU)."""
+    r = lib().bale_is_perm(_ptr(perm), perm.numel(), _stream(stream))
+    if r < 0:
+        check(r)
+    return r == 1

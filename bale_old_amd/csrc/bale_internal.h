@@ -116,6 +116,13 @@ int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins,
 int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
                         bool packed, const bale_config& c, Device* d, hipStream_t s);
 
+// random permutation by dart throwing (randperm.hip); darts_used (optional)
+// receives the darts generated
+int launch_randperm(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream_t s,
+                    int64_t* darts_used);
+// *ok = perm[0..N) holds 0..N-1 once each (waits for the stream)
+int launch_check_perm(const int64_t* perm, int64_t N, Device* d, hipStream_t s, int* ok);
+
 int launch_probe_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 int launch_probe_random(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
                         hipStream_t s);

@@ -582,3 +582,76 @@ double ig_exstack_pkg(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t
 }
 
 }  // extern "C"
+
+// ======================= random permutation ================================
+extern "C" {
+
+int bale_randperm(int64_t* perm, int64_t n, int64_t seed, void* stream) {
+  if (n < 0) return fail(BALE_ERR_ARG, "negative permutation length %lld", (long long)n);
+  if (n == 0) return BALE_OK;
+  if (!perm) return fail(BALE_ERR_ARG, "null permutation");
+  if (n >= ((int64_t)1 << 30))
+    return fail(BALE_ERR_ARG, "permutation length must be below 2^30 (slots 2n below 2^31)");
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  int rc = launch_randperm(perm, n, seed, d, (hipStream_t)stream, nullptr);
+  if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "workspace allocation failed");
+  return rc ? fail_hip(rc, "randperm") : BALE_OK;
+}
+
+int bale_is_perm(const int64_t* perm, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && !perm)) return fail(BALE_ERR_ARG, "bad permutation array");
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  int ok = 0;
+  int rc = launch_check_perm(perm, n, d, (hipStream_t)stream, &ok);
+  if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "workspace allocation failed");
+  if (rc) return fail_hip(rc, "is_perm");
+  return ok;
+}
+
+// The reference's models of rand_permp (spmat/spmat.h:115-118) all return a
+// permutation in the shared heap; here it is device memory from
+// bale_all_alloc, filled by the dart-throwing AGI model (the result the
+// reference's rand_permp_agi gives at one PE), and the GPU seconds go to
+// bale_hip_last_timing.  NULL on error.
+static int64_t* randperm_dropin(int64_t N, int seed) {
+  t_gpu_s = t_h2d_s = t_d2h_s = 0.0;
+  if (N < 0 || N >= ((int64_t)1 << 30)) {
+    fail(BALE_ERR_ARG, "permutation length %lld outside [0, 2^30)", (long long)N);
+    return nullptr;
+  }
+  int64_t* perm = (int64_t*)bale_all_alloc(N ? N : 1, sizeof(int64_t));
+  if (!perm) return nullptr;
+  Device* d = device_state();
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, 0);
+  int rc = N ? launch_randperm(perm, N, seed, d, 0, nullptr) : 0;
+  hipEventRecord(e1, 0);
+  if (!rc) rc = (int)hipEventSynchronize(e1);
+  if (!rc) t_gpu_s = elapsed(e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) {
+    bale_all_free(perm);
+    if (rc == hipErrorOutOfMemory) fail(BALE_ERR_ALLOC, "workspace allocation failed");
+    else fail_hip(rc, "randperm");
+    return nullptr;
+  }
+  return perm;
+}
+
+int64_t* rand_permp_agi(int64_t N, int seed) { return randperm_dropin(N, seed); }
+int64_t* rand_permp_exstack(int64_t N, int seed, int64_t) { return randperm_dropin(N, seed); }
+int64_t* rand_permp_exstack2(int64_t N, int seed, int64_t) { return randperm_dropin(N, seed); }
+int64_t* rand_permp_conveyor(int64_t N, int seed) { return randperm_dropin(N, seed); }
+
+// spmat/spmat.h:152 -- 1 if perm (device memory) holds 0..N-1 once each.
+int is_perm(int64_t* perm, int64_t N) {
+  int r = bale_is_perm(perm, N, nullptr);
+  return r > 0 ? 1 : 0;
+}
+
+}  // extern "C"

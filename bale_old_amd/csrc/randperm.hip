@@ -1,0 +1,311 @@
+// randperm.hip -- random permutation of 0..N-1 by dart throwing
+// (spmat/spmat_agi.upc:56-107, rand_permp_agi).
+//
+// At one PE the reference is sequential: darts r = lrand48() % M (M = 2N) are
+// thrown in turn, and the i-th dart that finds its slot empty places item i;
+// the permutation lists the filled slots in order.  The outcome has a parallel
+// statement: dart k hits iff no earlier dart chose its slot, and the item it
+// places is the number of hits before it (placing stops at N items).  So:
+//   k_darts    each dart's slot, from the drand48 sequence (jump-ahead per thread)
+//   k_first    first[slot] = smallest dart index choosing it (device atomic min)
+//   k_hits     darts that are their slot's first, flagged (bit 31), counted per tile
+//   k_place    first[slot] = the hit's item (or empty from item N on)
+//   k_occ / k_compact   filled slots counted per tile, then written in slot order
+// with k_scan_tiles turning per-tile counts into offsets.  Every step is exact,
+// so the result equals the reference's run at one PE for the same seed.
+#include <algorithm>
+#include <cmath>
+
+#include "bale_internal.h"
+
+namespace bale {
+
+namespace rp {
+
+constexpr uint64_t kA = 0x5DEECE66DULL, kC = 0xBULL, kMask48 = (1ULL << 48) - 1;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kThreads = 1024;
+constexpr int kPer = 16;                     // elements per thread per tile
+constexpr int kTile = kThreads * kPer;       // 16384 elements per tile
+constexpr int kDartsPerThread = 16;          // consecutive darts per thread in k_darts
+
+// X -> a X + c (mod 2^48); composing two such maps of the same generator.
+struct Lcg {
+  uint64_t a, c;
+};
+__host__ __device__ inline Lcg lcg_then(Lcg f, Lcg g) {  // f, then g
+  return {(g.a * f.a) & kMask48, (g.a * f.c + g.c) & kMask48};
+}
+// The map of k steps of the generator.
+__host__ __device__ inline Lcg lcg_steps(uint64_t k) {
+  Lcg r{1, 0}, b{kA, kC};
+  while (k) {
+    if (k & 1) r = lcg_then(r, b);
+    b = lcg_then(b, b);
+    k >>= 1;
+  }
+  return r;
+}
+
+// slot[k] = (lrand48 value k) mod M: lrand48 returns the top 31 bits of the
+// state after k + 1 steps from x0.
+__global__ __launch_bounds__(256) void k_darts(uint64_t x0, uint64_t D, uint32_t M,
+                                               uint32_t* __restrict__ slot) {
+  const uint64_t k0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kDartsPerThread;
+  if (k0 >= D) return;
+  const Lcg j = lcg_steps(k0);
+  uint64_t x = (j.a * x0 + j.c) & kMask48;
+  uint32_t v[kDartsPerThread];
+#pragma unroll
+  for (int i = 0; i < kDartsPerThread; i++) {
+    x = (kA * x + kC) & kMask48;
+    v[i] = (uint32_t)(x >> 17) % M;
+  }
+  if (k0 + kDartsPerThread <= D) {
+    u32x4* o = (u32x4*)(slot + k0);  // k0 is a multiple of 16: 64-byte aligned
+#pragma unroll
+    for (int q = 0; q < kDartsPerThread / 4; q++) o[q] = u32x4{v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+  } else {
+    for (int i = 0; i < kDartsPerThread && k0 + i < D; i++) slot[k0 + i] = v[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ slot, uint64_t D,
+                                               uint32_t* __restrict__ first) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    __hip_atomic_fetch_min(first + slot[k], (uint32_t)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of one value per thread over a workgroup of kThreads; every thread gets it.
+__device__ inline uint32_t block_sum(uint32_t v, uint32_t* sh) {
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int w = 0; w < kThreads / 64; w++) t += sh[w];
+  return t;
+}
+
+// Hit darts of a tile (element e = tile*kTile + i*kThreads + tid): flag them in
+// bit 31 of their slot word and count them.
+__global__ __launch_bounds__(kThreads) void k_hits(uint32_t* __restrict__ slot, uint64_t D,
+                                                   const uint32_t* __restrict__ first,
+                                                   uint32_t* __restrict__ tile_count) {
+  __shared__ uint32_t sh[kThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint32_t n = 0;
+#pragma unroll 4
+  for (int i = 0; i < kPer; i++) {
+    const uint64_t k = base + (uint64_t)i * kThreads + threadIdx.x;
+    if (k < D) {
+      const uint32_t s = slot[k];
+      if (first[s] == (uint32_t)k) {
+        slot[k] = s | 0x80000000u;
+        n++;
+      }
+    }
+  }
+  const uint32_t t = block_sum(n, sh);
+  if (threadIdx.x == 0) tile_count[blockIdx.x] = t;
+}
+
+// Exclusive prefix of tile_count[0..nt) into tile_off, total in tile_off[nt].
+// One workgroup.
+__global__ __launch_bounds__(kThreads) void k_scan_tiles(const uint32_t* __restrict__ tile_count,
+                                                         uint32_t nt, uint32_t* __restrict__ tile_off) {
+  __shared__ uint32_t part[kThreads];
+  const int t = threadIdx.x;
+  const uint32_t per = (nt + kThreads - 1) / kThreads, t0 = t * per;
+  uint32_t sum = 0;
+  for (uint32_t i = t0; i < t0 + per && i < nt; i++) sum += tile_count[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < kThreads; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = t ? part[t - 1] : 0u;
+  for (uint32_t i = t0; i < t0 + per && i < nt; i++) {
+    tile_off[i] = run;
+    run += tile_count[i];
+  }
+  if (t == kThreads - 1) tile_off[nt] = part[kThreads - 1];
+}
+
+// Rank of this thread's flag among the tile's flags in element order (round i,
+// then thread); `round_base` carries the tile's flags of earlier rounds.
+__device__ inline uint32_t round_rank(bool f, uint32_t& round_base, uint32_t* sh) {
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint64_t bal = __ballot(f);
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+  __syncthreads();
+  if (lane == 0) sh[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (int w = 0; w < kThreads / 64; w++) {
+    const uint32_t c = sh[w];
+    before += w < wave ? c : 0u;
+    total += c;
+  }
+  const uint32_t r = round_base + before + below;
+  round_base += total;
+  return r;
+}
+
+// first[slot] = item of the slot's hit dart: its rank among all hits, or empty
+// from item N on (the sequential process stops after N hits).
+__global__ __launch_bounds__(kThreads) void k_place(const uint32_t* __restrict__ slot, uint64_t D,
+                                                    const uint32_t* __restrict__ tile_off,
+                                                    uint32_t N, uint32_t* __restrict__ first) {
+  __shared__ uint32_t sh[kThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint32_t rb = tile_off[blockIdx.x];
+  for (int i = 0; i < kPer; i++) {
+    const uint64_t k = base + (uint64_t)i * kThreads + threadIdx.x;
+    const uint32_t s = k < D ? slot[k] : 0u;
+    const bool hit = (s >> 31) != 0;
+    const uint32_t item = round_rank(hit, rb, sh);
+    if (hit) first[s & 0x7FFFFFFFu] = item < N ? item : kEmpty;
+  }
+}
+
+// Filled slots per tile.
+__global__ __launch_bounds__(kThreads) void k_occ(const uint32_t* __restrict__ first, uint64_t M,
+                                                  uint32_t* __restrict__ tile_count) {
+  __shared__ uint32_t sh[kThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint32_t n = 0;
+#pragma unroll 4
+  for (int i = 0; i < kPer; i++) {
+    const uint64_t s = base + (uint64_t)i * kThreads + threadIdx.x;
+    if (s < M && first[s] != kEmpty) n++;
+  }
+  const uint32_t t = block_sum(n, sh);
+  if (threadIdx.x == 0) tile_count[blockIdx.x] = t;
+}
+
+// perm[rank of filled slot s] = its item, in slot order.
+__global__ __launch_bounds__(kThreads) void k_compact(const uint32_t* __restrict__ first, uint64_t M,
+                                                      const uint32_t* __restrict__ tile_off,
+                                                      int64_t* __restrict__ perm) {
+  __shared__ uint32_t sh[kThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint32_t rb = tile_off[blockIdx.x];
+  for (int i = 0; i < kPer; i++) {
+    const uint64_t s = base + (uint64_t)i * kThreads + threadIdx.x;
+    const uint32_t v = s < M ? first[s] : kEmpty;
+    const bool f = v != kEmpty;
+    const uint32_t pos = round_rank(f, rb, sh);
+    if (f) perm[pos] = (int64_t)v;
+  }
+}
+
+// perm holds 0..N-1 once each: seen[] marks values; any duplicate or value out
+// of range sets *bad.
+__global__ __launch_bounds__(256) void k_check_perm(const int64_t* __restrict__ perm, uint64_t N,
+                                                    uint32_t* __restrict__ seen,
+                                                    uint32_t* __restrict__ bad) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int64_t v = perm[i];
+    if (v < 0 || (uint64_t)v >= N || atomicExch(seen + v, 1u) != 0u) atomicOr(bad, 1u);
+  }
+}
+
+}  // namespace rp
+
+// Darts for N hits: the expected count is M ln(M / (M - N)) = 2N ln 2 for
+// M = 2N, with a standard deviation below sqrt(N); eight of them plus a
+// constant leave a shortfall practically impossible, and one is handled by
+// starting again with twice the darts.
+static uint64_t randperm_darts(uint64_t N) {
+  return (uint64_t)(1.3862944 * (double)N + 8.0 * sqrt((double)N)) + 4096;
+}
+
+int launch_randperm(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream_t s,
+                    int64_t* darts_used) {
+  using namespace rp;
+  if (N <= 0) return 0;
+  const uint64_t M = 2 * (uint64_t)N;
+  // srand48(seed) when seed != 0, else drand48's initial state
+  const uint64_t x0 = seed != 0 ? ((uint64_t)(uint32_t)seed << 16) | 0x330EULL : 0x1234ABCD330EULL;
+  for (uint64_t D = randperm_darts((uint64_t)N);; D *= 2) {
+    if (D >= ((uint64_t)1 << 31)) return (int)hipErrorInvalidValue;
+    const uint64_t ntd = (D + kTile - 1) / kTile, nts = (M + kTile - 1) / kTile;
+    const uint64_t nt = ntd > nts ? ntd : nts;
+    auto up = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+    const uint64_t b_slot = up(D * 4), b_first = up(M * 4), b_cnt = up(nt * 4), b_off = up((nt + 1) * 4);
+    char* ws = (char*)workspace(d, b_slot + b_first + b_cnt + b_off + 256, s);
+    if (!ws) return (int)hipErrorOutOfMemory;
+    uint32_t* slot = (uint32_t*)ws;
+    uint32_t* first = (uint32_t*)(ws + b_slot);
+    uint32_t* cnt = (uint32_t*)(ws + b_slot + b_first);
+    uint32_t* off = (uint32_t*)(ws + b_slot + b_first + b_cnt);
+    const unsigned g_fill = (unsigned)std::min<uint64_t>(4096, (M + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g_fill), dim3(256), 0, s, first, M, kEmpty);
+    const uint64_t nthr_d = (D + kDartsPerThread - 1) / kDartsPerThread;
+    hipLaunchKernelGGL(k_darts, dim3((unsigned)((nthr_d + 255) / 256)), dim3(256), 0, s, x0, D,
+                       (uint32_t)M, slot);
+    const unsigned g_d = (unsigned)std::min<uint64_t>((uint64_t)d->cus * 32, (D + 255) / 256);
+    hipLaunchKernelGGL(k_first, dim3(g_d), dim3(256), 0, s, (const uint32_t*)slot, D, first);
+    hipLaunchKernelGGL(k_hits, dim3((unsigned)ntd), dim3(kThreads), 0, s, slot, D,
+                       (const uint32_t*)first, cnt);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, s, (const uint32_t*)cnt,
+                       (uint32_t)ntd, off);
+    uint32_t hits = 0;
+    int rc = (int)hipMemcpyAsync(&hits, off + ntd, 4, hipMemcpyDeviceToHost, s);
+    if (!rc) rc = (int)hipStreamSynchronize(s);
+    if (rc) return rc;
+    if (hits < (uint64_t)N) continue;  // too few darts: again with twice as many
+    hipLaunchKernelGGL(k_place, dim3((unsigned)ntd), dim3(kThreads), 0, s, (const uint32_t*)slot, D,
+                       (const uint32_t*)off, (uint32_t)N, first);
+    hipLaunchKernelGGL(k_occ, dim3((unsigned)nts), dim3(kThreads), 0, s, (const uint32_t*)first, M,
+                       cnt);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, s, (const uint32_t*)cnt,
+                       (uint32_t)nts, off);
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)nts), dim3(kThreads), 0, s, (const uint32_t*)first,
+                       M, (const uint32_t*)off, perm);
+    if (darts_used) *darts_used = (int64_t)D;
+    return (int)hipGetLastError();
+  }
+}
+
+int launch_check_perm(const int64_t* perm, int64_t N, Device* d, hipStream_t s, int* ok) {
+  using namespace rp;
+  *ok = 0;
+  if (N < 0) return (int)hipErrorInvalidValue;
+  if (N == 0) {
+    *ok = 1;
+    return 0;
+  }
+  const uint64_t bytes = (((uint64_t)N * 4 + 255) & ~(uint64_t)255) + 256;
+  char* ws = (char*)workspace(d, bytes, s);
+  if (!ws) return (int)hipErrorOutOfMemory;
+  uint32_t* seen = (uint32_t*)ws;
+  uint32_t* bad = (uint32_t*)(ws + bytes - 256);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<uint64_t>(4096, ((uint64_t)N + 255) / 256)),
+                     dim3(256), 0, s, seen, (uint64_t)N, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, bad, (uint64_t)1, 0u);
+  hipLaunchKernelGGL(k_check_perm, dim3((unsigned)std::min<uint64_t>((uint64_t)d->cus * 32, ((uint64_t)N + 255) / 256)),
+                     dim3(256), 0, s, perm, (uint64_t)N, seen, bad);
+  uint32_t b = 1;
+  int rc = (int)hipMemcpyAsync(&b, bad, 4, hipMemcpyDeviceToHost, s);
+  if (!rc) rc = (int)hipStreamSynchronize(s);
+  if (rc) return rc;
+  *ok = b == 0;
+  return 0;
+}
+
+}  // namespace bale

@@ -310,6 +310,35 @@ def cpu_convey(n=1 << 24, P=64):
     return {"n": n, "pes": P, "Gitems_per_s": n / t / 1e9, "cores": 1, "kind": "port"}
 
 
+def bench_randperm(B, torch, dev, stream, N=1 << 26, reps=5, seed=101892):
+    """rand_permp_agi at one PE on the GPU: 2^26 elements (about the reference
+    driver's 10^6 per thread at 64 threads), seed 101892 (its PE-0 seed)."""
+    perm = torch.empty(N, dtype=torch.int64, device=dev)
+    B.randperm(perm, seed, stream)
+    assert B.is_perm(perm, stream), "randperm output is not a permutation"
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        B.randperm(perm, seed, stream)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t = median(ts)
+    return {"n": N, "seed": seed, "median_s": t, "Gelems_per_s": N / t / 1e9,
+            "note": "wall time of one call (darts, first-dart table, hit ranks, placement, "
+                    "compaction; one 4-byte readback of the hit count)"}
+
+
+def cpu_randperm(N=1 << 24, seed=101892):
+    """The oracle's sequential dart loop (glibc lrand48), one host thread."""
+    import oracle
+    oracle.randperm_agi(1 << 16, seed)
+    t0 = time.perf_counter()
+    oracle.randperm_agi(N, seed)
+    t = time.perf_counter() - t0
+    return {"n": N, "Gelems_per_s": N / t / 1e9, "cores": 1, "kind": "port"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,6 +414,11 @@ def main() -> int:
         if not args.no_cpu:
             extra["convey"]["cpu_baseline"] = cpu_convey()
         log(f"conveyor: {extra['convey']}")
+        log("randperm 2^26 ...")
+        extra["randperm"] = bench_randperm(B, torch, dev, stream)
+        if not args.no_cpu:
+            extra["randperm"]["cpu_baseline"] = cpu_randperm()
+        log(f"randperm: {extra['randperm']}")
 
     if not args.no_batch:
         from bale_old_amd import selector as S

@@ -71,6 +71,20 @@ double ig_exstack2_goto(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64
 double ig_exstack_function(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable, int64_t buf_cnt);
 double ig_exstack_pkg(int64_t *tgt, int64_t *pckindx, int64_t l_num_req, int64_t *ltable, int64_t buf_cnt);
 
+/* spmat/spmat.h:115-118 -- a random permutation of 0..N-1 by dart throwing
+ * (spmat/spmat_agi.upc:56-107).  Returns device memory from bale_all_alloc
+ * (free with bale_all_free, as the reference frees with lgp_all_free), NULL on
+ * error; GPU seconds in bale_hip_last_timing.  Every model gives the result of
+ * the reference's rand_permp_agi at one PE for the same seed (the reference's
+ * exstack / exstack2 / conveyor models re-throw rejected darts in batches and
+ * so give other, equally valid, permutations).  N < 2^30; buf_cnt is ignored. */
+int64_t *rand_permp_agi(int64_t N, int seed);
+int64_t *rand_permp_exstack(int64_t N, int seed, int64_t buf_cnt);
+int64_t *rand_permp_exstack2(int64_t N, int seed, int64_t buf_cnt);
+int64_t *rand_permp_conveyor(int64_t N, int seed);
+/* spmat/spmat.h:152 -- 1 if perm (device memory) holds 0..N-1 exactly once. */
+int is_perm(int64_t *perm, int64_t N);
+
 /* ============================ 2. native API =========================== */
 
 #define BALE_OK          0
@@ -140,6 +154,14 @@ int bale_sync_errors(void *stream, int64_t *bad);
 /* The configuration BALE_STRAT_AUTO would run for this shape. */
 int bale_histo_plan(int64_t n, int64_t nbins, int dtype, bale_config *out);
 int bale_gather_plan(int64_t n, int64_t table_len, bale_config *out);
+
+/* perm[0..n) (device memory) = the random permutation rand_permp_agi gives
+ * at one PE for `seed` (seed 0: drand48's initial state, i.e. no srand48).
+ * n < 2^30.  Waits for `stream` once, to check that the darts sufficed. */
+int bale_randperm(int64_t *perm, int64_t n, int64_t seed, void *stream);
+/* 1 if perm[0..n) (device memory) holds 0..n-1 exactly once, 0 if not,
+ * negative BALE_ERR_* on error; waits for `stream`. */
+int bale_is_perm(const int64_t *perm, int64_t n, void *stream);
 
 /* ---- device memory for C callers ------------------------------------- */
 /* libgetput/libgetput.h:159 (lgp_all_alloc) at one PE: zeroed device memory

@@ -58,6 +58,11 @@ def lib():
             "or_batch_params": (None, [P, I, U, C.c_int, C.c_int, C.c_int, C.c_int]),
             "or_digest": (U, [P, I]),
             "or_convey_deliver": (I, [P, P, I, I, I, P, P]),
+            "or_srand48_state": (U, [I]),
+            "or_lrand48_seq": (None, [U, I, P]),
+            "or_glibc_lrand48": (None, [I, I, P]),
+            "or_randperm_agi": (I, [P, I, I]),
+            "or_is_perm": (C.c_int, [P, I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -189,3 +194,32 @@ def convey_deliver(items: np.ndarray, pes: np.ndarray, P: int) -> tuple[np.ndarr
 
 def digest(v: np.ndarray) -> int:
     return lib().or_digest(_p(np.ascontiguousarray(v, dtype=np.int64)), v.size)
+
+
+def srand48_state(seed: int) -> int:
+    return lib().or_srand48_state(seed)
+
+
+def lrand48_seq(state: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.int64)
+    lib().or_lrand48_seq(state, n, _p(out))
+    return out
+
+
+def glibc_lrand48(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.int64)
+    lib().or_glibc_lrand48(seed, n, _p(out))
+    return out
+
+
+def randperm_agi(N: int, seed: int) -> tuple[np.ndarray, int]:
+    """(permutation, darts thrown) of rand_permp_agi at one PE."""
+    perm = np.empty(max(N, 1), np.int64)
+    darts = lib().or_randperm_agi(_p(perm), N, seed)
+    assert darts >= 0
+    return perm[:N], darts
+
+
+def is_perm(perm: np.ndarray, N: int) -> bool:
+    perm = np.ascontiguousarray(perm, dtype=np.int64)
+    return bool(lib().or_is_perm(_p(perm), N)) if perm.size == N else False

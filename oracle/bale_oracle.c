@@ -370,3 +370,68 @@ uint64_t or_digest(const int64_t *v, int64_t n) {
   for (int64_t i = 0; i < n; i++) acc += mix64((uint64_t)v[i] ^ (uint64_t)i);
   return acc;
 }
+
+/* ---- random permutation by dart throwing ------------------------------ */
+/* The drand48 family's generator (POSIX): X <- (0x5DEECE66D X + 0xB) mod 2^48,
+ * lrand48 returns the top 31 bits of the new X, and srand48(s) sets
+ * X = (s mod 2^32) << 16 | 0x330E.  Restated from the published definition so
+ * a GPU sequence can be checked element by element; or_randperm_agi below calls
+ * glibc's own lrand48, as the reference does. */
+#define LCG48_A 0x5DEECE66DULL
+#define LCG48_C 0xBULL
+#define LCG48_MASK ((1ULL << 48) - 1)
+
+uint64_t or_srand48_state(int64_t seed) {
+  return ((uint64_t)(uint32_t)seed << 16) | 0x330EULL;
+}
+
+void or_lrand48_seq(uint64_t state, int64_t n, int64_t *out) {
+  uint64_t x = state & LCG48_MASK;
+  for (int64_t i = 0; i < n; i++) {
+    x = (LCG48_A * x + LCG48_C) & LCG48_MASK;
+    out[i] = (int64_t)(x >> 17);
+  }
+}
+
+void or_glibc_lrand48(int64_t seed, int64_t n, int64_t *out) {
+  srand48((long)seed);
+  for (int64_t i = 0; i < n; i++) out[i] = lrand48();
+}
+
+/* rand_permp_agi at one PE (spmat/spmat_agi.upc:56-107): darts r = lrand48() % M,
+ * M = 2N, thrown in turn; the i-th dart that finds target[r] empty places item
+ * i there (:81-87, one PE: item i*THREADS + MYTHREAD = i); the permutation lists
+ * the filled slots in order (:90-101).  srand48(seed) only when seed != 0 (:61),
+ * on glibc's process-wide state, as in the reference.  Returns the number of
+ * darts thrown, or -1 when memory runs out. */
+int64_t or_randperm_agi(int64_t *perm, int64_t N, int64_t seed) {
+  const int64_t M = 2 * N;
+  int64_t *target = (int64_t *)malloc((size_t)(M ? M : 1) * sizeof(int64_t));
+  if (!target) return -1;
+  for (int64_t i = 0; i < M; i++) target[i] = -1;
+  if (seed != 0) srand48((long)(int)seed);
+  int64_t darts = 0;
+  for (int64_t i = 0; i < N;) {
+    const int64_t r = lrand48() % M;
+    darts++;
+    if (target[r] == -1) target[r] = i++;
+  }
+  int64_t pos = 0;
+  for (int64_t s = 0; s < M; s++)
+    if (target[s] != -1) perm[pos++] = target[s];
+  free(target);
+  return darts;
+}
+
+/* is_perm (spmat/spmat_utils.upc:508-530): every value 0..N-1 exactly once. */
+int or_is_perm(const int64_t *perm, int64_t N) {
+  unsigned char *seen = (unsigned char *)calloc((size_t)(N ? N : 1), 1);
+  if (!seen) return 0;
+  int ok = 1;
+  for (int64_t i = 0; i < N && ok; i++) {
+    if (perm[i] < 0 || perm[i] >= N || seen[perm[i]]) ok = 0;
+    else seen[perm[i]] = 1;
+  }
+  free(seen);
+  return ok;
+}

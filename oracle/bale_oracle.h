@@ -115,6 +115,19 @@ void or_batch_params(or_instance *out, int64_t count, uint64_t seed,
 int64_t or_convey_deliver(const void *items, const int64_t *pes, int64_t n, int64_t item_bytes,
                           int64_t P, void *out, int64_t *starts);
 
+/* ---------------- random permutation (dart throwing) ------------------ */
+/* drand48 generator state after srand48(seed): (seed mod 2^32) << 16 | 0x330E. */
+uint64_t or_srand48_state(int64_t seed);
+/* n successive lrand48 values from a 48-bit state, restated from POSIX. */
+void or_lrand48_seq(uint64_t state, int64_t n, int64_t *out);
+/* glibc's own srand48(seed) then n calls of lrand48. */
+void or_glibc_lrand48(int64_t seed, int64_t n, int64_t *out);
+/* rand_permp_agi at one PE (spmat/spmat_agi.upc:56-107) into perm[0..N);
+ * returns the darts thrown (-1: out of memory). */
+int64_t or_randperm_agi(int64_t *perm, int64_t N, int64_t seed);
+/* is_perm (spmat/spmat_utils.upc:508): 1 if perm holds 0..N-1 once each. */
+int or_is_perm(const int64_t *perm, int64_t N);
+
 /* Order-independent 64-bit digest of an int64 array (sum of mix(v_i ^ i)). */
 uint64_t or_digest(const int64_t *v, int64_t n);
 

@@ -1,0 +1,168 @@
+"""Random permutation by dart throwing (rand_permp_agi, spmat/spmat_agi.upc:56-107).
+
+CPU: the oracle's drand48 restatement against glibc's lrand48 and the published
+generator; the oracle's sequential dart loop against an independent Python
+loop; and the parallel statement the GPU computes (a dart hits iff it is the
+first to choose its slot; its item is the number of hits before it) against
+the sequential oracle.  GPU: the library's permutation equal to the oracle's
+element for element, through the native API and the reference's drop-ins.
+"""
+import numpy as np
+import pytest
+
+import bale_old_amd as B
+
+A48, C48, MASK48 = 0x5DEECE66D, 0xB, (1 << 48) - 1
+INIT_STATE = 0x1234ABCD330E  # drand48's state when srand48 was never called
+
+
+def py_lrand48(state, n):
+    out = []
+    for _ in range(n):
+        state = (A48 * state + C48) & MASK48
+        out.append(state >> 17)
+    return out
+
+
+def py_randperm_sequential(N, state):
+    """The reference loop at one PE, with the generator restated in Python."""
+    M = 2 * N
+    target = [-1] * M
+    i = 0
+    while i < N:
+        state = (A48 * state + C48) & MASK48
+        r = (state >> 17) % M
+        if target[r] == -1:
+            target[r] = i
+            i += 1
+    return [t for t in target if t != -1]
+
+
+def np_randperm_parallel(N, state, orc, darts):
+    """What the GPU computes, restated with numpy on the oracle's sequence."""
+    M = 2 * N
+    slot = orc.lrand48_seq(state, darts) % M
+    # first dart of every slot, in dart order
+    _, first_idx = np.unique(slot, return_index=True)
+    hit = np.zeros(darts, bool)
+    hit[first_idx] = True
+    item = np.cumsum(hit) - 1
+    target = np.full(M, -1, np.int64)
+    keep = hit & (item < N)
+    target[slot[keep]] = item[keep]
+    assert keep.sum() == N
+    return target[target != -1]
+
+
+# ---- CPU: the oracle pinned ------------------------------------------------------
+@pytest.mark.parametrize("seed", [101892, 1, 0x7FFFFFFF, -5, 123456789012])
+def test_lrand48_restatement_matches_glibc(orc, seed):
+    want = orc.glibc_lrand48(seed, 1000)
+    got = orc.lrand48_seq(orc.srand48_state(seed), 1000)
+    assert np.array_equal(got, want)
+    # srand48 keeps the low 32 bits of the seed and appends 0x330E
+    assert orc.srand48_state(seed) == ((seed & 0xFFFFFFFF) << 16) | 0x330E
+    assert list(got[:20]) == py_lrand48(orc.srand48_state(seed), 20)
+
+
+def test_lrand48_published_first_values():
+    # X1 = (0x5DEECE66D * 0x330E + 0xB) mod 2^48 after srand48(0); lrand48 = X1 >> 17
+    x1 = (0x5DEECE66D * 0x330E + 0xB) % (1 << 48)
+    assert py_lrand48(0x330E, 1)[0] == x1 >> 17 == 366850414
+
+
+@pytest.mark.parametrize("N,seed", [(1, 3), (2, 101892), (7, 11), (100, 101892), (1000, 77)])
+def test_oracle_matches_python_sequential(orc, N, seed):
+    perm, darts = orc.randperm_agi(N, seed)
+    want = py_randperm_sequential(N, orc.srand48_state(seed))
+    assert perm.tolist() == want
+    assert orc.is_perm(perm, N)
+    assert darts >= N
+
+
+@pytest.mark.parametrize("N,seed", [(1, 5), (3, 9), (1000, 101892), (100_000, 42)])
+def test_parallel_statement_equals_sequential(orc, N, seed):
+    perm, darts = orc.randperm_agi(N, seed)
+    # the same number of darts the sequential run threw, and more: darts past
+    # the N-th hit place nothing
+    for extra in (0, 5, 1000):
+        got = np_randperm_parallel(N, orc.srand48_state(seed), orc, darts + extra)
+        assert np.array_equal(got, perm), extra
+
+
+def test_dart_count_within_the_planned_margin(orc):
+    # the GPU generates 2N ln 2 + 8 sqrt(N) + 4096 darts: far above any run
+    for N, seed in ((10_000, 1), (300_000, 2), (1_000_000, 101892)):
+        _, darts = orc.randperm_agi(N, seed)
+        assert abs(darts - 1.3862944 * N) < 4 * np.sqrt(N) + 100, (N, darts)
+
+
+def test_oracle_is_perm_detects_errors(orc):
+    p = np.arange(10, dtype=np.int64)
+    assert orc.is_perm(p, 10)
+    q = p.copy()
+    q[3] = 4
+    assert not orc.is_perm(q, 10)
+    q = p.copy()
+    q[9] = 10
+    assert not orc.is_perm(q, 10)
+
+
+# ---- GPU -----------------------------------------------------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", [1, 2, 3, 1000, 12_345, 1 << 20])
+@pytest.mark.parametrize("seed", [101892, 7, -3])
+def test_randperm_matches_oracle(dev, orc, N, seed):
+    import torch
+    perm = torch.full((N,), -9, dtype=torch.int64, device=dev)
+    B.randperm(perm, seed)
+    want, _ = orc.randperm_agi(N, seed)
+    assert np.array_equal(perm.cpu().numpy(), want)
+    assert B.is_perm(perm)
+
+
+@pytest.mark.gpu
+def test_randperm_seed0_uses_the_initial_state(dev, orc):
+    import torch
+    N = 5000
+    perm = torch.empty(N, dtype=torch.int64, device=dev)
+    B.randperm(perm, 0)
+    want = np.array(py_randperm_sequential(N, INIT_STATE), dtype=np.int64)
+    assert np.array_equal(perm.cpu().numpy(), want)
+
+
+@pytest.mark.gpu
+def test_randperm_dropins_and_is_perm(dev, orc):
+    L = B.lib()
+    N, seed = 100_000, 101892  # the reference driver's seed at PE 0 (randperm.upc:107)
+    want, _ = orc.randperm_agi(N, seed)
+    for name, extra in (("rand_permp_agi", ()), ("rand_permp_exstack", (1024,)),
+                        ("rand_permp_exstack2", (1024,)), ("rand_permp_conveyor", ())):
+        p = getattr(L, name)(N, seed, *extra)
+        assert p, B.last_error()
+        gpu_s, _, _ = B.last_timing()
+        assert gpu_s > 0
+        host = np.empty(N, np.int64)
+        assert L.bale_memget(host.ctypes.data, p, N * 8) == 0
+        assert np.array_equal(host, want), name
+        assert L.is_perm(p, N) == 1
+        L.bale_all_free(p)
+    assert L.rand_permp_agi(-1, 1) is None
+    assert "outside" in B.last_error()
+
+
+@pytest.mark.gpu
+def test_is_perm_on_gpu_detects_errors(dev):
+    import torch
+    p = torch.arange(1000, dtype=torch.int64, device=dev)
+    assert B.is_perm(p)
+    q = p.clone()
+    q[10] = 11
+    assert not B.is_perm(q)
+    q = p.clone()
+    q[5] = -1
+    assert not B.is_perm(q)
+    q = p.clone()
+    q[0] = 1000
+    assert not B.is_perm(q)
+    assert B.is_perm(torch.empty(0, dtype=torch.int64, device=dev))
