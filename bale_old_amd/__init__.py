@@ -63,6 +63,12 @@ class Instance(C.Structure):
     ]
 
 
+class SparseMat(C.Structure):
+    """The reference's sparsemat_t (spmat/spmat.h:73-86), field for field."""
+    _fields_ = [(n, C.c_int64) for n in ("local", "numrows", "lnumrows", "numcols", "nnz", "lnnz")] + [
+        (n, C.c_void_p) for n in ("offset", "loffset", "nonzero", "lnonzero")]
+
+
 class BaleError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"bale error {code}: {msg}")
@@ -118,6 +124,26 @@ def _proto(L) -> None:
     for name in ("rand_permp_exstack", "rand_permp_exstack2"):
         f(name, VP, I64T, C.c_int, I64T)
     f("is_perm", C.c_int, VP, I64T)
+    f("rand_permp", VP, I64T, C.c_int)
+    f("bale_spmat_transpose", C.c_int, I64T, I64T, I64T, VP, VP, VP, VP, VP)
+    f("bale_spmat_permute", C.c_int, I64T, I64T, I64T, VP, VP, VP, VP, VP, VP, VP)
+    f("bale_spmat_gen_er", C.c_int, I64T, C.c_double, C.c_int, C.c_int, C.c_uint64, VP, VP, I64T, P64, VP)
+    f("bale_spmat_triangular", C.c_int, I64T, VP, VP, C.c_int, P64, P64, VP)
+    SP = C.POINTER(SparseMat)
+    for name in ("transpose_matrix_agi", "transpose_matrix_conveyor", "transpose_matrix"):
+        f(name, SP, SP)
+    for name in ("transpose_matrix_exstack", "transpose_matrix_exstack2"):
+        f(name, SP, SP, I64T)
+    for name in ("permute_matrix_agi", "permute_matrix_conveyor", "permute_matrix"):
+        f(name, SP, SP, VP, VP)
+    for name in ("permute_matrix_exstack", "permute_matrix_exstack2"):
+        f(name, SP, SP, VP, VP, I64T)
+    f("gen_erdos_renyi_graph_dist", SP, C.c_int, C.c_double, I64T, I64T, I64T)
+    f("init_matrix", SP, I64T, I64T, I64T)
+    f("clear_matrix", None, SP)
+    f("compare_matrix", C.c_int, SP, SP)
+    f("is_upper_triangular", C.c_int, SP, I64T)
+    f("is_lower_triangular", C.c_int, SP, I64T)
     f("bale_all_alloc", VP, I64T, I64T)
     f("bale_all_free", None, VP)
     f("bale_memput", C.c_int, VP, VP, I64T)
@@ -259,3 +285,56 @@ def is_perm(perm, stream=None) -> bool:
     if r < 0:
         check(r)
     return r == 1
+
+
+# ---- sparse matrices (CSR as device int64 tensors) -------------------------------------
+def spmat_gen_er(n: int, p: float, unit_diag: int = 0, mode: int = 3, seed: int = 1, stream=None):
+    """(offset, nonzero) of an Erdos-Renyi matrix generated on the GPU
+    (gen_erdos_renyi_graph_dist's method, spmat/spmat_utils.upc:554)."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    off = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    nnz = C.c_int64(0)
+    check(lib().bale_spmat_gen_er(n, p, unit_diag, mode, seed, _ptr(off), None, 0, C.byref(nnz),
+                                  _stream(stream)))
+    nz = torch.empty(max(nnz.value, 1), dtype=torch.int64, device=dev)
+    check(lib().bale_spmat_gen_er(n, p, unit_diag, mode, seed, _ptr(off), _ptr(nz), nz.numel(),
+                                  C.byref(nnz), _stream(stream)))
+    return off, nz[: nnz.value]
+
+
+def spmat_transpose(numcols: int, offset, nonzero, t_offset=None, t_nonzero=None, stream=None):
+    """(t_offset, t_nonzero): the transpose (transpose_matrix_agi, spmat/spmat_agi.upc:176)."""
+    import torch
+    numrows = offset.numel() - 1
+    if t_offset is None:
+        t_offset = torch.empty(numcols + 1, dtype=torch.int64, device=offset.device)
+    if t_nonzero is None:
+        t_nonzero = torch.empty(max(nonzero.numel(), 1), dtype=torch.int64, device=offset.device)
+    check(lib().bale_spmat_transpose(numrows, numcols, nonzero.numel(), _ptr(offset),
+                                     _ptr(nonzero) if nonzero.numel() else None, _ptr(t_offset),
+                                     _ptr(t_nonzero), _stream(stream)))
+    return t_offset, t_nonzero[: nonzero.numel()]
+
+
+def spmat_permute(numcols: int, offset, nonzero, rperminv, cperminv, stream=None):
+    """(p_offset, p_nonzero): row i to row rperminv[i], column j to cperminv[j]
+    (permute_matrix_agi, spmat/spmat_agi.upc:119)."""
+    import torch
+    numrows = offset.numel() - 1
+    p_offset = torch.empty(numrows + 1, dtype=torch.int64, device=offset.device)
+    p_nonzero = torch.empty(max(nonzero.numel(), 1), dtype=torch.int64, device=offset.device)
+    check(lib().bale_spmat_permute(numrows, numcols, nonzero.numel(), _ptr(offset),
+                                   _ptr(nonzero) if nonzero.numel() else None, _ptr(rperminv),
+                                   _ptr(cperminv), _ptr(p_offset), _ptr(p_nonzero), _stream(stream)))
+    return p_offset, p_nonzero[: nonzero.numel()]
+
+
+def spmat_triangular(offset, nonzero, upper: bool, stream=None) -> tuple[int, int]:
+    """(entries on the wrong side of the diagonal, rows without a diagonal entry)."""
+    wrong, nopivot = C.c_int64(0), C.c_int64(0)
+    check(lib().bale_spmat_triangular(offset.numel() - 1, _ptr(offset),
+                                      _ptr(nonzero) if nonzero.numel() else None, int(upper),
+                                      C.byref(wrong), C.byref(nopivot), _stream(stream)))
+    return wrong.value, nopivot.value
+

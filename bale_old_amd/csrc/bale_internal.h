@@ -123,6 +123,21 @@ int launch_randperm(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream
 // *ok = perm[0..N) holds 0..N-1 once each (waits for the stream)
 int launch_check_perm(const int64_t* perm, int64_t N, Device* d, hipStream_t s, int* ok);
 
+// sparse matrices (spmat.hip)
+int launch_spmat_gen_er(int64_t n, double p, int unit_diag, int mode, uint64_t seed,
+                        int64_t* offset, int64_t* nonzero, Device* d, hipStream_t s);
+int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const int64_t* offset,
+                           const int64_t* nonzero, int64_t* t_offset, int64_t* t_nonzero,
+                           Device* d, hipStream_t s);
+int launch_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const int64_t* offset,
+                         const int64_t* nonzero, const int64_t* rperminv, const int64_t* cperminv,
+                         int64_t* p_offset, int64_t* p_nonzero, Device* d, hipStream_t s);
+int launch_spmat_triangular(const int64_t* offset, const int64_t* nonzero, int64_t numrows, int upper,
+                            Device* d, hipStream_t s, int64_t* wrong, int64_t* nopivot);
+// *ndiff = entries where a and b differ (waits for the stream)
+int launch_count_diff(const int64_t* a, const int64_t* b, int64_t n, Device* d, hipStream_t s,
+                      int64_t* ndiff);
+
 int launch_probe_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 int launch_probe_random(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
                         hipStream_t s);

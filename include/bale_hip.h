@@ -84,6 +84,71 @@ int64_t *rand_permp_exstack2(int64_t N, int seed, int64_t buf_cnt);
 int64_t *rand_permp_conveyor(int64_t N, int seed);
 /* spmat/spmat.h:152 -- 1 if perm (device memory) holds 0..N-1 exactly once. */
 int is_perm(int64_t *perm, int64_t N);
+/* spmat/spmat_utils.upc:92-107 -- the reference's wrapper (its default model, AGI). */
+int64_t *rand_permp(int64_t N, int seed);
+
+/* Sparse matrices: the reference's sparsemat_t (spmat/spmat.h:73-86), field
+ * for field.  At one PE the local and global views coincide: lnumrows ==
+ * numrows, lnnz == nnz, loffset == offset, lnonzero == nonzero.  Matrices the
+ * library makes hold device memory (bale_all_alloc); inputs may hold device or
+ * host memory (host arrays are staged).  Row i's column indices are
+ * nonzero[offset[i] .. offset[i+1]). */
+typedef struct sparsemat_t {
+  int64_t local;
+  int64_t numrows;
+  int64_t lnumrows;
+  int64_t numcols;
+  int64_t nnz;
+  int64_t lnnz;
+  int64_t *offset;
+  int64_t *loffset;
+  int64_t *nonzero;
+  int64_t *lnonzero;
+} sparsemat_t;
+
+/* spmat/spmat.h:125-128 (spmat/spmat_agi.upc:176-236) -- the transpose; every
+ * row of it lists its entries in increasing row of A, as the reference's AGI
+ * model does at one PE.  The four models give the same matrix; buf_cnt is
+ * ignored.  NULL on error (bale_hip_last_error).  Needs numrows, numcols below
+ * 2^32 and nnz below 2^31. */
+sparsemat_t *transpose_matrix_agi(sparsemat_t *A);
+sparsemat_t *transpose_matrix_exstack(sparsemat_t *A, int64_t buf_cnt);
+sparsemat_t *transpose_matrix_exstack2(sparsemat_t *A, int64_t buf_cnt);
+sparsemat_t *transpose_matrix_conveyor(sparsemat_t *A);
+/* spmat/spmat_utils.upc:109-127 -- the reference's wrapper. */
+sparsemat_t *transpose_matrix(sparsemat_t *A);
+/* spmat/spmat.h:120-123 (spmat/spmat_agi.upc:119-169) -- row i of A becomes
+ * row rperminv[i], column j becomes column cperminv[j]; rows keep the order of
+ * their entries.  rperminv / cperminv: device memory (as rand_permp returns). */
+sparsemat_t *permute_matrix_agi(sparsemat_t *A, int64_t *rperminv, int64_t *cperminv);
+sparsemat_t *permute_matrix_exstack(sparsemat_t *A, int64_t *rperminv, int64_t *cperminv,
+                                    int64_t buf_cnt);
+sparsemat_t *permute_matrix_exstack2(sparsemat_t *A, int64_t *rperminv, int64_t *cperminv,
+                                     int64_t buf_cnt);
+sparsemat_t *permute_matrix_conveyor(sparsemat_t *A, int64_t *rperminv, int64_t *cperminv);
+/* spmat/spmat_utils.upc:92-107 -- the reference's wrapper. */
+sparsemat_t *permute_matrix(sparsemat_t *A, int64_t *rperminv, int64_t *cperminv);
+/* spmat/spmat_utils.upc:554-610 -- an Erdos-Renyi matrix of n rows, edge
+ * probability p, by geometric gaps between nonzeros (the reference's method);
+ * mode 1 lower triangle, 2 upper, 3 both halves (asymmetric); the diagonal
+ * when unit_diag.  The gaps come from splitmix64 streams per (row, half), not
+ * from the reference's rand() sequence, so a seed gives another matrix of the
+ * same distribution.  Mode 0 (symmetric) is not provided.  Device memory. */
+sparsemat_t *gen_erdos_renyi_graph_dist(int n, double p, int64_t unit_diag, int64_t mode,
+                                        int64_t seed);
+/* spmat/spmat_utils.upc:1153-1183 -- zeroed device arrays: numrows + 1
+ * offsets, nnz nonzeros. */
+sparsemat_t *init_matrix(int64_t numrows, int64_t numcols, int64_t nnz);
+/* spmat/spmat_utils.upc:1211-1220 -- frees the arrays (not the struct). */
+void clear_matrix(sparsemat_t *mat);
+/* spmat/spmat_utils.upc:1070-1125 -- 0 when equal (sizes, offsets, nonzeros),
+ * 1 otherwise; compared on the GPU. */
+int compare_matrix(sparsemat_t *lmat, sparsemat_t *rmat);
+/* spmat/spmat_utils.upc:377-453 -- 1 when no entry lies below (upper) / above
+ * (lower) the diagonal and, with unit_diagonal, every row has its diagonal
+ * entry; 0 otherwise. */
+int is_upper_triangular(sparsemat_t *A, int64_t unit_diagonal);
+int is_lower_triangular(sparsemat_t *A, int64_t unit_diagonal);
 
 /* ============================ 2. native API =========================== */
 
@@ -162,6 +227,28 @@ int bale_randperm(int64_t *perm, int64_t n, int64_t seed, void *stream);
 /* 1 if perm[0..n) (device memory) holds 0..n-1 exactly once, 0 if not,
  * negative BALE_ERR_* on error; waits for `stream`. */
 int bale_is_perm(const int64_t *perm, int64_t n, void *stream);
+
+/* Sparse matrices as device arrays (offset[numrows + 1], nonzero[nnz]).  Each
+ * call finishes on `stream` before it returns (it reports range errors).
+ * Transpose: t_offset[numcols + 1], t_nonzero[nnz]; numrows, numcols < 2^32,
+ * nnz < 2^31. */
+int bale_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const int64_t *offset,
+                         const int64_t *nonzero, int64_t *t_offset, int64_t *t_nonzero,
+                         void *stream);
+/* Permute: p_offset[numrows + 1], p_nonzero[nnz]; rperminv[numrows],
+ * cperminv[numcols] permutations. */
+int bale_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const int64_t *offset,
+                       const int64_t *nonzero, const int64_t *rperminv, const int64_t *cperminv,
+                       int64_t *p_offset, int64_t *p_nonzero, void *stream);
+/* Erdos-Renyi rows (as gen_erdos_renyi_graph_dist): offset[n + 1] is always
+ * written and *nnz set; nonzero (capacity nnz_cap) is filled when not NULL and
+ * large enough (else BALE_ERR_ARG). */
+int bale_spmat_gen_er(int64_t n, double p, int unit_diag, int mode, uint64_t seed, int64_t *offset,
+                      int64_t *nonzero, int64_t nnz_cap, int64_t *nnz, void *stream);
+/* *wrong = entries on the wrong side of the diagonal, *nopivot = rows without
+ * their diagonal entry. */
+int bale_spmat_triangular(int64_t numrows, const int64_t *offset, const int64_t *nonzero,
+                          int upper, int64_t *wrong, int64_t *nopivot, void *stream);
 
 /* ---- device memory for C callers ------------------------------------- */
 /* libgetput/libgetput.h:159 (lgp_all_alloc) at one PE: zeroed device memory

@@ -63,6 +63,10 @@ def lib():
             "or_glibc_lrand48": (None, [I, I, P]),
             "or_randperm_agi": (I, [P, I, I]),
             "or_is_perm": (C.c_int, [P, I]),
+            "or_det_log": (C.c_double, [C.c_double]),
+            "or_gen_er": (I, [I, C.c_double, C.c_int, C.c_int, U, P, P]),
+            "or_transpose_agi": (None, [I, I, P, P, P, P]),
+            "or_permute_agi": (None, [I, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -223,3 +227,37 @@ def randperm_agi(N: int, seed: int) -> tuple[np.ndarray, int]:
 def is_perm(perm: np.ndarray, N: int) -> bool:
     perm = np.ascontiguousarray(perm, dtype=np.int64)
     return bool(lib().or_is_perm(_p(perm), N)) if perm.size == N else False
+
+
+def det_log(x: float) -> float:
+    return lib().or_det_log(x)
+
+
+def gen_er(n: int, p: float, unit_diag: int = 0, mode: int = 3, seed: int = 1):
+    """(offset, nonzero) of the Erdos-Renyi matrix the library generates."""
+    off = np.empty(n + 1, np.int64)
+    nnz = lib().or_gen_er(n, p, unit_diag, mode, seed, _p(off), None)
+    nz = np.empty(max(nnz, 1), np.int64)
+    lib().or_gen_er(n, p, unit_diag, mode, seed, _p(off), _p(nz))
+    return off, nz[:nnz]
+
+
+def transpose_agi(numrows: int, numcols: int, off: np.ndarray, nz: np.ndarray):
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    nz = np.ascontiguousarray(nz, dtype=np.int64)
+    toff = np.empty(numcols + 1, np.int64)
+    tnz = np.empty(max(nz.size, 1), np.int64)
+    lib().or_transpose_agi(numrows, numcols, _p(off), _p(nz), _p(toff), _p(tnz))
+    return toff, tnz[: nz.size]
+
+
+def permute_agi(off: np.ndarray, nz: np.ndarray, rperminv: np.ndarray, cperminv: np.ndarray):
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    nz = np.ascontiguousarray(nz, dtype=np.int64)
+    rperminv = np.ascontiguousarray(rperminv, dtype=np.int64)
+    cperminv = np.ascontiguousarray(cperminv, dtype=np.int64)
+    numrows = off.size - 1
+    poff = np.empty(numrows + 1, np.int64)
+    pnz = np.empty(max(nz.size, 1), np.int64)
+    lib().or_permute_agi(numrows, _p(off), _p(nz), _p(rperminv), _p(cperminv), _p(poff), _p(pnz))
+    return poff, pnz[: nz.size]

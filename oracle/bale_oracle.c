@@ -423,6 +423,115 @@ int64_t or_randperm_agi(int64_t *perm, int64_t N, int64_t seed) {
   return darts;
 }
 
+/* ---- sparse matrices ------------------------------------------------------ */
+
+/* log(x), x in (0, 1]: x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m =
+ * 2 atanh(s), s = (m - 1)/(m + 1), |s| < 0.172, by its series to s^25
+ * (remainder below 2^-60 relative); plus e ln 2.  Only correctly rounded
+ * operations (the Makefile turns contraction off), so any IEEE double
+ * implementation gives the same bits. */
+double or_det_log(double x) {
+  uint64_t b;
+  memcpy(&b, &x, 8);
+  int64_t e = (int64_t)((b >> 52) & 0x7FF) - 1022;
+  b = (b & 0x000FFFFFFFFFFFFFULL) | (0x3FEULL << 52); /* m in [0.5, 1) */
+  double m;
+  memcpy(&m, &b, 8);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0), z = s * s;
+  double t = 1.0 / 25.0;
+  for (int k = 11; k >= 0; k--) t = t * z + 1.0 / (double)(2 * k + 1);
+  return (double)e * 0.69314718055994530942 + 2.0 * s * t;
+}
+
+/* the generator's stream for (row, half) and its geometric gaps */
+static uint64_t er_key(uint64_t seed, int64_t row, int half) {
+  return or_splitmix64(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(2 * row + half + 1));
+}
+/* nonzeros of row `row` in columns [lo, hi), ascending; written to out when
+ * not NULL; returns the count */
+static int64_t er_half(uint64_t key, int64_t lo, int64_t hi, double p, double D, int64_t *out) {
+  int64_t cnt = 0;
+  if (lo >= hi || p <= 0.0) return 0;
+  if (p >= 1.0) {
+    for (int64_t c = lo; c < hi; c++) {
+      if (out) out[cnt] = c;
+      cnt++;
+    }
+    return cnt;
+  }
+  int64_t col = lo - 1;
+  for (uint64_t i = 0;; i++) {
+    const uint64_t x = or_splitmix64_at(key, i);
+    const double u = (double)((x >> 11) + 1) * 0x1p-53; /* (0, 1] */
+    const double g = floor(or_det_log(u) / D);
+    if (g >= (double)(hi - col)) break;
+    col += 1 + (int64_t)g;
+    if (col >= hi) break;
+    if (out) out[cnt] = col;
+    cnt++;
+  }
+  return cnt;
+}
+
+int64_t or_gen_er(int64_t n, double p, int unit_diag, int mode, uint64_t seed, int64_t *offset,
+                  int64_t *nonzero) {
+  const double D = p > 0.0 && p < 1.0 ? or_det_log(1.0 - p) : -1.0;
+  int64_t pos = 0;
+  offset[0] = 0;
+  for (int64_t r = 0; r < n; r++) {
+    if (mode == 1 || mode == 3)
+      pos += er_half(er_key(seed, r, 0), 0, r, p, D, nonzero ? nonzero + pos : NULL);
+    if (unit_diag) {
+      if (nonzero) nonzero[pos] = r;
+      pos++;
+    }
+    if (mode == 2 || mode == 3)
+      pos += er_half(er_key(seed, r, 1), r + 1, n, p, D, nonzero ? nonzero + pos : NULL);
+    offset[r + 1] = pos;
+  }
+  return pos;
+}
+
+void or_transpose_agi(int64_t numrows, int64_t numcols, const int64_t *offset,
+                      const int64_t *nonzero, int64_t *t_offset, int64_t *t_nonzero) {
+  const int64_t nnz = offset[numrows];
+  int64_t *cur = (int64_t *)calloc((size_t)(numcols ? numcols : 1), sizeof(int64_t));
+  /* column counts (:196-201, the fetch_and_inc histogram) */
+  for (int64_t j = 0; j < nnz; j++) cur[nonzero[j]]++;
+  /* row starts of the transpose (:216-218) */
+  t_offset[0] = 0;
+  for (int64_t c = 0; c < numcols; c++) {
+    t_offset[c + 1] = t_offset[c] + cur[c];
+    cur[c] = t_offset[c];
+  }
+  /* rows in order, each nonzero to the next free slot of its column (:225-230) */
+  for (int64_t r = 0; r < numrows; r++)
+    for (int64_t j = offset[r]; j < offset[r + 1]; j++) t_nonzero[cur[nonzero[j]]++] = r;
+  free(cur);
+}
+
+void or_permute_agi(int64_t numrows, const int64_t *offset, const int64_t *nonzero,
+                    const int64_t *rperminv, const int64_t *cperminv, int64_t *p_offset,
+                    int64_t *p_nonzero) {
+  int64_t *rperm = (int64_t *)malloc((size_t)(numrows ? numrows : 1) * sizeof(int64_t));
+  /* rperm from rperminv (:128-131) */
+  for (int64_t i = 0; i < numrows; i++) rperm[rperminv[i]] = i;
+  /* permuted rows in order (:145-154), then the columns mapped (:158-163) */
+  int64_t pos = 0;
+  p_offset[0] = 0;
+  for (int64_t i = 0; i < numrows; i++) {
+    const int64_t row = rperm[i];
+    for (int64_t j = offset[row]; j < offset[row + 1]; j++) p_nonzero[pos++] = nonzero[j];
+    p_offset[i + 1] = pos;
+  }
+  for (int64_t j = 0; j < pos; j++) p_nonzero[j] = cperminv[p_nonzero[j]];
+  free(rperm);
+}
+
 /* is_perm (spmat/spmat_utils.upc:508-530): every value 0..N-1 exactly once. */
 int or_is_perm(const int64_t *perm, int64_t N) {
   unsigned char *seen = (unsigned char *)calloc((size_t)(N ? N : 1), 1);

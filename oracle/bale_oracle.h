@@ -128,6 +128,30 @@ int64_t or_randperm_agi(int64_t *perm, int64_t N, int64_t seed);
 /* is_perm (spmat/spmat_utils.upc:508): 1 if perm holds 0..N-1 once each. */
 int or_is_perm(const int64_t *perm, int64_t N);
 
+/* ---------------- sparse matrices (CSR at one PE) ---------------------- */
+/* A matrix is offset[numrows + 1] (offset[0] = 0) and nonzero[offset[numrows]]:
+ * row i's column indices are nonzero[offset[i] .. offset[i+1]). */
+
+/* Natural logarithm of x in (0, 1] by the atanh series, in plain IEEE double
+ * operations (no fused multiply-add) so the GPU computes the same bits. */
+double or_det_log(double x);
+/* Erdos-Renyi matrix of n rows with edge probability p drawn by geometric gaps
+ * between nonzeros (Batagelj-Brandes; the reference uses the same method,
+ * spmat/spmat_utils.upc:612-700), with splitmix64 streams per row and half:
+ * mode 1 lower triangle, 2 upper, 3 both (asymmetric); the diagonal when
+ * unit_diag.  offset always filled; nonzero written when not NULL.  Returns
+ * the number of nonzeros. */
+int64_t or_gen_er(int64_t n, double p, int unit_diag, int mode, uint64_t seed, int64_t *offset,
+                  int64_t *nonzero);
+/* transpose_matrix_agi at one PE (spmat/spmat_agi.upc:176-236). */
+void or_transpose_agi(int64_t numrows, int64_t numcols, const int64_t *offset,
+                      const int64_t *nonzero, int64_t *t_offset, int64_t *t_nonzero);
+/* permute_matrix_agi at one PE (spmat/spmat_agi.upc:119-169): row i of A goes
+ * to row rperminv[i], column j to column cperminv[j]. */
+void or_permute_agi(int64_t numrows, const int64_t *offset, const int64_t *nonzero,
+                    const int64_t *rperminv, const int64_t *cperminv, int64_t *p_offset,
+                    int64_t *p_nonzero);
+
 /* Order-independent 64-bit digest of an int64 array (sum of mix(v_i ^ i)). */
 uint64_t or_digest(const int64_t *v, int64_t n);
 
