@@ -142,6 +142,11 @@ def _proto(L) -> None:
     f("init_matrix", SP, I64T, I64T, I64T)
     f("clear_matrix", None, SP)
     f("compare_matrix", C.c_int, SP, SP)
+    f("bale_toposort", C.c_int, I64T, VP, VP, VP, VP, VP, VP, P64, VP)
+    for name in ("toposort_matrix_agi", "toposort_matrix_exstack", "toposort_matrix_exstack2",
+                 "toposort_matrix_convey", "toposort_matrix_oo", "toposort_matrix_cooler",
+                 "toposort_matrix_exstack_orig"):
+        f(name, C.c_double, VP, VP, SP, SP)
     f("is_upper_triangular", C.c_int, SP, I64T)
     f("is_lower_triangular", C.c_int, SP, I64T)
     f("bale_all_alloc", VP, I64T, I64T)
@@ -337,4 +342,18 @@ def spmat_triangular(offset, nonzero, upper: bool, stream=None) -> tuple[int, in
                                       _ptr(nonzero) if nonzero.numel() else None, int(upper),
                                       C.byref(wrong), C.byref(nopivot), _stream(stream)))
     return wrong.value, nopivot.value
+
+
+def toposort(offset, nonzero, t_offset, t_nonzero, stream=None):
+    """(rperm, cperm, levels) for a morally upper-triangular matrix and its
+    transpose (toposort_matrix_agi, apps/topo_src/toposort_agi.upc:53)."""
+    import torch
+    n = offset.numel() - 1
+    rperm = torch.empty(max(n, 1), dtype=torch.int64, device=offset.device)
+    cperm = torch.empty(max(n, 1), dtype=torch.int64, device=offset.device)
+    lev = C.c_int64(0)
+    check(lib().bale_toposort(n, _ptr(offset), _ptr(nonzero) if nonzero.numel() else None,
+                              _ptr(t_offset), _ptr(t_nonzero) if t_nonzero.numel() else None,
+                              _ptr(rperm), _ptr(cperm), C.byref(lev), _stream(stream)))
+    return rperm[:n], cperm[:n], lev.value
 

@@ -134,6 +134,11 @@ int launch_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const in
                          int64_t* p_offset, int64_t* p_nonzero, Device* d, hipStream_t s);
 int launch_spmat_triangular(const int64_t* offset, const int64_t* nonzero, int64_t numrows, int upper,
                             Device* d, hipStream_t s, int64_t* wrong, int64_t* nopivot);
+// toposort (toposort.hip): waits for the stream; pivots found, levels, the
+// count of column indices out of range and of columns claimed twice
+int launch_toposort(int64_t n, const int64_t* offset, const int64_t* nonzero, const int64_t* t_offset,
+                    const int64_t* t_nonzero, int64_t* rperm, int64_t* cperm, Device* d, hipStream_t s,
+                    int64_t* pivots, int64_t* levels, int64_t* bad_col, int64_t* twice);
 // *ndiff = entries where a and b differ (waits for the stream)
 int launch_count_diff(const int64_t* a, const int64_t* b, int64_t n, Device* d, hipStream_t s,
                       int64_t* ndiff);

@@ -718,6 +718,28 @@ int bale_spmat_gen_er(int64_t n, double p, int unit_diag, int mode, uint64_t see
   return rc ? spmat_rc(rc, "gen_er") : BALE_OK;
 }
 
+int bale_toposort(int64_t n, const int64_t* offset, const int64_t* nonzero, const int64_t* t_offset,
+                  const int64_t* t_nonzero, int64_t* rperm, int64_t* cperm, int64_t* levels,
+                  void* stream) {
+  if (n < 0 || n >= ((int64_t)1 << 31)) return fail(BALE_ERR_ARG, "toposort: n outside [0, 2^31)");
+  if (n && (!offset || !nonzero || !t_offset || !t_nonzero || !rperm || !cperm))
+    return fail(BALE_ERR_ARG, "toposort: null array");
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  int64_t piv = 0, lev = 0, bad_col = 0, twice = 0;
+  int rc = launch_toposort(n, offset, nonzero, t_offset, t_nonzero, rperm, cperm, d,
+                           (hipStream_t)stream, &piv, &lev, &bad_col, &twice);
+  if (rc) return spmat_rc(rc, "toposort");
+  if (levels) *levels = lev;
+  if (bad_col) return fail(BALE_ERR_RANGE, "toposort: %lld column indices outside [0, %lld)",
+                           (long long)bad_col, (long long)n);
+  if (twice || piv != n)
+    return fail(BALE_ERR_ARG, "toposort: %lld pivots for %lld rows, %lld columns claimed twice: "
+                "the matrix is not morally upper triangular", (long long)piv, (long long)n,
+                (long long)twice);
+  return BALE_OK;
+}
+
 int bale_spmat_triangular(int64_t numrows, const int64_t* offset, const int64_t* nonzero, int upper,
                           int64_t* wrong, int64_t* nopivot, void* stream) {
   if (numrows < 0 || !offset || !wrong || !nopivot) return fail(BALE_ERR_ARG, "bad arguments");
@@ -849,6 +871,22 @@ static sparsemat_t* permute_dropin(sparsemat_t* A, const int64_t* rperminv, cons
   return Ap;
 }
 
+static double toposort_dropin(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
+  if (!matrix_ok(mat) || !matrix_ok(tmat)) return -1.0;
+  if (mat->numrows != mat->numcols || tmat->numrows != mat->numcols || tmat->nnz != mat->nnz) {
+    fail(BALE_ERR_ARG, "toposort: mat must be square and tmat its transpose");
+    return -1.0;
+  }
+  StagedI64 off, nz, toff, tnz;
+  if (off.stage(mat->offset, mat->numrows + 1) || nz.stage(mat->nonzero, mat->nnz) ||
+      toff.stage(tmat->offset, tmat->numrows + 1) || tnz.stage(tmat->nonzero, tmat->nnz))
+    return -1.0;
+  DropinClock clk;
+  int rc = bale_toposort(mat->numrows, off.p, nz.p, toff.p, tnz.p, rperm, cperm, nullptr, nullptr);
+  clk.stop();
+  return rc ? -1.0 : t_gpu_s;
+}
+
 static int triangular_dropin(sparsemat_t* A, int64_t unit_diagonal, int upper) {
   if (!matrix_ok(A)) return 0;
   StagedI64 off, nz;
@@ -947,6 +985,29 @@ int compare_matrix(sparsemat_t* lmat, sparsemat_t* rmat) {
   if (d1) return 1;
   if (launch_count_diff(ln.p, rn.p, lmat->nnz, d, 0, &d2)) return 1;
   return d2 ? 1 : 0;
+}
+
+double toposort_matrix_agi(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
+  return toposort_dropin(rperm, cperm, mat, tmat);
+}
+double toposort_matrix_exstack(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
+  return toposort_dropin(rperm, cperm, mat, tmat);
+}
+double toposort_matrix_exstack2(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
+  return toposort_dropin(rperm, cperm, mat, tmat);
+}
+double toposort_matrix_convey(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
+  return toposort_dropin(rperm, cperm, mat, tmat);
+}
+double toposort_matrix_oo(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
+  return toposort_dropin(rperm, cperm, mat, tmat);
+}
+double toposort_matrix_cooler(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
+  return toposort_dropin(rperm, cperm, mat, tmat);
+}
+double toposort_matrix_exstack_orig(int64_t* rperm, int64_t* cperm, sparsemat_t* mat,
+                                    sparsemat_t* tmat) {
+  return toposort_dropin(rperm, cperm, mat, tmat);
 }
 
 int is_upper_triangular(sparsemat_t* A, int64_t unit_diagonal) {

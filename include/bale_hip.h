@@ -150,6 +150,24 @@ int compare_matrix(sparsemat_t *lmat, sparsemat_t *rmat);
 int is_upper_triangular(sparsemat_t *A, int64_t unit_diagonal);
 int is_lower_triangular(sparsemat_t *A, int64_t unit_diagonal);
 
+/* apps/topo_src/toposort.h:47-50 and alternates/toposort_alternates.h:42-44
+ * (apps/topo_src/toposort_agi.upc:53-166) -- rperm[row], cperm[col] (device
+ * memory, numrows entries each) such that permute_matrix(mat, rperm, cperm)
+ * is upper triangular with a full diagonal; mat is a row and column
+ * permutation of such a matrix and tmat its transpose.  Every model returns
+ * the permutations toposort_matrix_agi finds at one PE (the other models
+ * process each level's queue in another order and may find other valid
+ * ones).  Returns GPU seconds, or -1.0 when mat is not morally triangular or
+ * on error (bale_hip_last_error). */
+double toposort_matrix_agi(int64_t *rperm, int64_t *cperm, sparsemat_t *mat, sparsemat_t *tmat);
+double toposort_matrix_exstack(int64_t *rperm, int64_t *cperm, sparsemat_t *mat, sparsemat_t *tmat);
+double toposort_matrix_exstack2(int64_t *rperm, int64_t *cperm, sparsemat_t *mat, sparsemat_t *tmat);
+double toposort_matrix_convey(int64_t *rperm, int64_t *cperm, sparsemat_t *mat, sparsemat_t *tmat);
+double toposort_matrix_oo(int64_t *rperm, int64_t *cperm, sparsemat_t *mat, sparsemat_t *tmat);
+double toposort_matrix_cooler(int64_t *rperm, int64_t *cperm, sparsemat_t *mat, sparsemat_t *tmat);
+double toposort_matrix_exstack_orig(int64_t *rperm, int64_t *cperm, sparsemat_t *mat,
+                                    sparsemat_t *tmat);
+
 /* ============================ 2. native API =========================== */
 
 #define BALE_OK          0
@@ -245,6 +263,13 @@ int bale_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const int6
  * large enough (else BALE_ERR_ARG). */
 int bale_spmat_gen_er(int64_t n, double p, int unit_diag, int mode, uint64_t seed, int64_t *offset,
                       int64_t *nonzero, int64_t nnz_cap, int64_t *nnz, void *stream);
+/* Toposort of an n x n morally upper-triangular matrix (offset, nonzero) given
+ * its transpose (t_offset, t_nonzero): rperm[n], cperm[n] as
+ * toposort_matrix_agi; *levels = the levels of the queue (may be NULL).
+ * BALE_ERR_ARG when the matrix is not morally triangular. */
+int bale_toposort(int64_t n, const int64_t *offset, const int64_t *nonzero, const int64_t *t_offset,
+                  const int64_t *t_nonzero, int64_t *rperm, int64_t *cperm, int64_t *levels,
+                  void *stream);
 /* *wrong = entries on the wrong side of the diagonal, *nopivot = rows without
  * their diagonal entry. */
 int bale_spmat_triangular(int64_t numrows, const int64_t *offset, const int64_t *nonzero,

@@ -67,6 +67,7 @@ def lib():
             "or_gen_er": (I, [I, C.c_double, C.c_int, C.c_int, U, P, P]),
             "or_transpose_agi": (None, [I, I, P, P, P, P]),
             "or_permute_agi": (None, [I, P, P, P, P, P, P]),
+            "or_toposort_agi": (I, [I, P, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -261,3 +262,18 @@ def permute_agi(off: np.ndarray, nz: np.ndarray, rperminv: np.ndarray, cperminv:
     pnz = np.empty(max(nz.size, 1), np.int64)
     lib().or_permute_agi(numrows, _p(off), _p(nz), _p(rperminv), _p(cperminv), _p(poff), _p(pnz))
     return poff, pnz[: nz.size]
+
+
+def toposort_agi(off: np.ndarray, nz: np.ndarray, toff: np.ndarray, tnz: np.ndarray):
+    """(rperm, cperm, pivots, levels) of toposort_matrix_agi at one PE."""
+    n = off.size - 1
+    rperm = np.full(max(n, 1), -1, np.int64)
+    cperm = np.full(max(n, 1), -1, np.int64)
+    lev = np.zeros(1, np.int64)
+    piv = lib().or_toposort_agi(n, _p(np.ascontiguousarray(off, dtype=np.int64)),
+                                _p(np.ascontiguousarray(nz, dtype=np.int64)),
+                                _p(np.ascontiguousarray(toff, dtype=np.int64)),
+                                _p(np.ascontiguousarray(tnz, dtype=np.int64)), _p(rperm), _p(cperm),
+                                _p(lev))
+    return rperm[:n], cperm[:n], piv, int(lev[0])
+

@@ -532,6 +532,49 @@ void or_permute_agi(int64_t numrows, const int64_t *offset, const int64_t *nonze
   free(rperm);
 }
 
+int64_t or_toposort_agi(int64_t n, const int64_t *offset, const int64_t *nonzero,
+                        const int64_t *t_offset, const int64_t *t_nonzero, int64_t *rperm,
+                        int64_t *cperm, int64_t *levels) {
+  const size_t sz = (size_t)(n ? n : 1) * sizeof(int64_t);
+  int64_t *queue = (int64_t *)malloc(sz), *rowsum = (int64_t *)malloc(sz),
+          *rowcnt = (int64_t *)malloc(sz);
+  /* rowcnt, rowsum (the sum of a row's columns) and the degree-one rows (:79-87) */
+  int64_t start = 0, end = 0, tail = 0;
+  for (int64_t i = 0; i < n; i++) {
+    rowcnt[i] = offset[i + 1] - offset[i];
+    rowsum[i] = 0;
+    for (int64_t j = offset[i]; j < offset[i + 1]; j++) rowsum[i] += nonzero[j];
+    if (rowcnt[i] == 1) queue[end++] = i;
+  }
+  tail = end;
+  /* levels (:110-138): the rows queued when a level starts are processed in
+   * order; each claims the next position from the end, and the rows of its
+   * column lose one nonzero; a row left with one joins the next level */
+  int64_t pos = 0, nlev = 0;
+  while (end > start) {
+    nlev++;
+    while (start < end) {
+      const int64_t row = queue[start++];
+      const int64_t col = rowsum[row];
+      rperm[row] = n - 1 - pos;
+      cperm[col] = n - 1 - pos;
+      pos++;
+      for (int64_t j = t_offset[col]; j < t_offset[col + 1]; j++) {
+        const int64_t s = t_nonzero[j];
+        const int64_t old = rowcnt[s]--;
+        rowsum[s] -= col;
+        if (old == 2) queue[tail++] = s;
+      }
+    }
+    end = tail;
+  }
+  if (levels) *levels = nlev;
+  free(queue);
+  free(rowsum);
+  free(rowcnt);
+  return pos;
+}
+
 /* is_perm (spmat/spmat_utils.upc:508-530): every value 0..N-1 exactly once. */
 int or_is_perm(const int64_t *perm, int64_t N) {
   unsigned char *seen = (unsigned char *)calloc((size_t)(N ? N : 1), 1);

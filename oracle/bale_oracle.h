@@ -152,6 +152,15 @@ void or_permute_agi(int64_t numrows, const int64_t *offset, const int64_t *nonze
                     const int64_t *rperminv, const int64_t *cperminv, int64_t *p_offset,
                     int64_t *p_nonzero);
 
+/* toposort_matrix_agi at one PE (apps/topo_src/toposort_agi.upc:53-166): mat
+ * a permuted upper-triangular n x n matrix, tmat its transpose (rows sorted).
+ * Writes rperm[row] and cperm[col] (the permutations that make mat upper
+ * triangular) and, when `levels` is not NULL, the number of levels.  Returns
+ * the pivots found (n on success). */
+int64_t or_toposort_agi(int64_t n, const int64_t *offset, const int64_t *nonzero,
+                        const int64_t *t_offset, const int64_t *t_nonzero, int64_t *rperm,
+                        int64_t *cperm, int64_t *levels);
+
 /* Order-independent 64-bit digest of an int64 array (sum of mix(v_i ^ i)). */
 uint64_t or_digest(const int64_t *v, int64_t n);
 
