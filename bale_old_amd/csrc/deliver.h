@@ -1,7 +1,7 @@
 // deliver.h -- stable delivery of items to destinations (one GPU).
 //
-// A chunk of items is cut into tiles; each tile's items are ranked per
-// destination with wave ballots, the virtual waves are prefixed in input
+// A chunk of items is cut into tiles; each wave ranks its consecutive items
+// of a tile per destination with ballots, the waves are prefixed in input
 // order, the tile is staged in LDS in destination order and each
 // destination's run is written to consecutive output slots.  Per-(chunk,
 // destination) counts from a count pass (rows) and the destinations' first
@@ -62,22 +62,23 @@ __device__ inline void item_store(uint32_t* __restrict__ dst, size_t i, const ui
 template <int W>
 constexpr int deliver_u() { return W <= 4 ? 4 : 2; }
 
-// LDS of k_convey_deliver<W>: per virtual wave (U x 16 of them) and PE a
-// count, then an offset; PE totals, tile starts and output cursors; the
-// staged items and their PEs.
+// LDS of k_convey_deliver<W>: per wave and PE a count, then an offset; PE
+// totals, tile starts and output cursors; the staged items and their PEs.
 template <int W>
 inline size_t deliver_lds(int npe) {
   constexpr int U = deliver_u<W>();
-  size_t words = (size_t)U * kDlvWaves * npe + 3 * (size_t)kMaxPes + 1;
+  size_t words = (size_t)kDlvWaves * npe + 3 * (size_t)kMaxPes + 1;
   words = (words + 3) & ~(size_t)3;  // 16-byte alignment for the staged items
   return words * 4 + (size_t)U * kDlvTile * W * 4 + (size_t)U * kDlvTile * 2;
 }
 
 // Stable delivery of chunk blockIdx.x.  W: 32-bit words per item.  A tile is
-// U x 1024 consecutive items, item base + u * 1024 + tid on thread tid; push
-// order within the tile is (u, wave, lane), so a "virtual wave" u * 16 + wave
-// ranks its items with ballots and the virtual waves are prefixed in order.
-// rows[c][p]: items of PE p in chunks before c; start[p]: PE p's first slot.
+// U x 1024 consecutive items; wave w owns items w * 64U .. w * 64U + 64U - 1
+// of it, 64 per round (item u * 64 + lane of the wave in round u).  A wave
+// ranks each round's items per PE with ballots on top of its running count
+// per PE (LDS), and the waves are prefixed in order, so the ranks follow push
+// order.  rows[c][p]: items of PE p in chunks before c; start[p]: PE p's
+// first slot.
 // KEY 0: destinations from `pes`; KEY 1: destination = (word 0 >> shift) &
 // (npe - 1), npe a power of two (`pes` unused); KEY 2: as 1, and the item is
 // written as the 64-bit value of its word 1 (W = 2: the last radix pass
@@ -92,27 +93,29 @@ __global__ __launch_bounds__(kDlvTile) void k_convey_deliver(const uint32_t* __r
                                                           uint32_t* __restrict__ out,
                                                           int shift = 0) {
   static_assert(KEY != 2 || W == 2, "KEY 2 writes word 1 of two-word items");
-  constexpr int U = deliver_u<W>(), V = U * kDlvWaves;
+  constexpr int U = deliver_u<W>();
   extern __shared__ __align__(16) uint32_t lds[];
-  uint32_t* wcnt = lds;                           // [V * npe]: count, then offset
-  uint32_t* tot = wcnt + (size_t)V * npe;         // [kMaxPes] tile items per PE
-  uint32_t* tstart = tot + kMaxPes;               // [kMaxPes + 1] first staged slot per PE
-  uint32_t* cursor = tstart + kMaxPes + 1;        // [kMaxPes] next output slot per PE
-  size_t head = ((size_t)V * npe + 3 * (size_t)kMaxPes + 1 + 3) & ~(size_t)3;
-  uint32_t* stage = lds + head;                   // [U * kDlvTile * W]
+  uint32_t* wcnt = lds;                                  // [kDlvWaves * npe]: count, then offset
+  uint32_t* tot = wcnt + (size_t)kDlvWaves * npe;        // [kMaxPes] tile items per PE
+  uint32_t* tstart = tot + kMaxPes;                      // [kMaxPes + 1] first staged slot per PE
+  uint32_t* cursor = tstart + kMaxPes + 1;               // [kMaxPes] next output slot per PE
+  size_t head = ((size_t)kDlvWaves * npe + 3 * (size_t)kMaxPes + 1 + 3) & ~(size_t)3;
+  uint32_t* stage = lds + head;                          // [U * kDlvTile * W]
   uint16_t* sdst = (uint16_t*)(stage + (size_t)U * kDlvTile * W);  // [U * kDlvTile]
   const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
   const int64_t c = blockIdx.x;
   const int64_t lo = c * chunk, hi = lo + chunk < n ? lo + chunk : n;
   for (int p = tid; p < npe; p += kDlvTile) cursor[p] = start[p] + rows[(size_t)c * npe + p];
   const uint64_t below = lane ? (~0ULL >> (64 - lane)) : 0ULL;  // lanes before this one
+  const int64_t mine = (int64_t)wave * (64 * U) + lane;           // this lane's first item in a tile
+  uint32_t* wc = wcnt + (size_t)wave * npe;
   // a tile's PE numbers and items are loaded while the previous tile is
   // ranked and written
   int64_t pn[U];
   uint32_t vn[U][W];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const int64_t e = lo + (int64_t)u * kDlvTile + tid;
+    const int64_t e = lo + mine + (int64_t)u * 64;
     pn[u] = -1;
     if (e < hi) {
       item_load<W>(items, (size_t)e, vn[u]);
@@ -128,16 +131,17 @@ __global__ __launch_bounds__(kDlvTile) void k_convey_deliver(const uint32_t* __r
       d[u] = (uint64_t)pn[u] < (uint64_t)npe ? (int)pn[u] : npe;
 #pragma unroll
       for (int k = 0; k < W; k++) v[u][k] = vn[u][k];
-      const int64_t e = base + (int64_t)(U + u) * kDlvTile + tid;
+      const int64_t e = base + (int64_t)U * kDlvTile + mine + (int64_t)u * 64;
       pn[u] = -1;
       if (e < hi) {
         item_load<W>(items, (size_t)e, vn[u]);
         pn[u] = KEY ? (int64_t)((vn[u][0] >> shift) & (uint32_t)(npe - 1)) : pes[e];
       }
     }
-    for (int k = tid; k < V * npe; k += kDlvTile) wcnt[k] = 0;
+    for (int k = tid; k < kDlvWaves * npe; k += kDlvTile) wcnt[k] = 0;
     __syncthreads();
-    // lanes of this wave with the same PE, by one ballot per bit of d
+    // round by round: lanes of this wave with the same PE (one ballot per bit
+    // of d), ranked after the wave's earlier items of that PE
     uint32_t rank[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -148,14 +152,17 @@ __global__ __launch_bounds__(kDlvTile) void k_convey_deliver(const uint32_t* __r
         const uint64_t b = __ballot(one);
         peers &= one ? b : ~b;
       }
-      rank[u] = (uint32_t)__popcll(peers & below);
-      if (has && rank[u] == 0) wcnt[(u * kDlvWaves + wave) * npe + d[u]] = (uint32_t)__popcll(peers);
+      const uint32_t r = (uint32_t)__popcll(peers & below);
+      const uint32_t before = has ? wc[d[u]] : 0u;  // every lane reads before the leader adds
+      rank[u] = before + r;
+      if (has && r == 0) wc[d[u]] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    // offsets of each virtual wave's items within the PE's part of the tile
+    // offsets of each wave's items within the PE's part of the tile
     for (int p = tid; p < npe; p += kDlvTile) {
       uint32_t run = 0;
-      for (int w = 0; w < V; w++) {
+#pragma unroll
+      for (int w = 0; w < kDlvWaves; w++) {
         const uint32_t t = wcnt[w * npe + p];
         wcnt[w * npe + p] = run;
         run += t;
@@ -190,10 +197,10 @@ __global__ __launch_bounds__(kDlvTile) void k_convey_deliver(const uint32_t* __r
 #pragma unroll
     for (int u = 0; u < U; u++)
       if (d[u] < npe) {
-        const uint32_t q = tstart[d[u]] + wcnt[(u * kDlvWaves + wave) * npe + d[u]] + rank[u];
+        const uint32_t q = tstart[d[u]] + wc[d[u]] + rank[u];
         if constexpr (KEY == 2) {
-          const uint32_t r[2] = {v[u][1], 0u};
-          item_store<W>(stage, q, r);
+          const uint32_t r2[2] = {v[u][1], 0u};
+          item_store<W>(stage, q, r2);
         } else {
           item_store<W>(stage, q, v[u]);
         }

@@ -339,6 +339,89 @@ def cpu_randperm(N=1 << 24, seed=101892):
     return {"n": N, "Gelems_per_s": N / t / 1e9, "cores": 1, "kind": "port"}
 
 
+def bench_spmat(B, torch, dev, stream, reps=3):
+    """Sparse-matrix kernels on the GPU, wall time of one call each (inputs
+    resident): transpose and permute of an Erdos-Renyi matrix of 2^24 rows
+    (the transpose driver's density: p = 18 / n, both triangles), and
+    toposort of the toposort driver's input at 2^20 rows (upper triangle with
+    a full diagonal, 35 per row, rows and columns permuted)."""
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return median(ts), r
+    out = {}
+    n = 1 << 24
+    off, nz = B.spmat_gen_er(n, 18 / n, 0, 3, seed=101894)
+    nnz = nz.numel()
+    t, (toff, tnz) = timed(lambda: B.spmat_transpose(n, off, nz))
+    ooff, onz = B.spmat_transpose(n, toff, tnz)
+    assert bool((ooff == off).all()) and bool((onz == nz).all()), "transpose of the transpose differs"
+    out["transpose"] = {"rows": n, "nnz": nnz, "median_s": t, "Gnnz_per_s": nnz / t / 1e9}
+    del toff, tnz, ooff, onz
+    rpi = torch.empty(n, dtype=torch.int64, device=dev)
+    cpi = torch.empty(n, dtype=torch.int64, device=dev)
+    B.randperm(rpi, 1230)
+    B.randperm(cpi, 45)
+    t, _ = timed(lambda: B.spmat_permute(n, off, nz, rpi, cpi))
+    out["permute"] = {"rows": n, "nnz": nnz, "median_s": t, "Gnnz_per_s": nnz / t / 1e9}
+    del off, nz, rpi, cpi
+    torch.cuda.empty_cache()
+    n = 1 << 20
+    off, nz = B.spmat_gen_er(n, 70 / (n - 1), 1, 2, seed=5)
+    rpi = torch.empty(n, dtype=torch.int64, device=dev)
+    cpi = torch.empty(n, dtype=torch.int64, device=dev)
+    B.randperm(rpi, 1230)
+    B.randperm(cpi, 45)
+    moff, mnz = B.spmat_permute(n, off, nz, rpi, cpi)
+    toff, tnz = B.spmat_transpose(n, moff, mnz)
+    t, (rp, cp, lev) = timed(lambda: B.toposort(moff, mnz, toff, tnz))
+    po, pn = B.spmat_permute(n, moff, mnz, rp, cp)
+    assert B.spmat_triangular(po, pn, True) == (0, 0), "toposort result is not triangular"
+    out["toposort"] = {"rows": n, "nnz": mnz.numel(), "levels": lev, "median_s": t,
+                       "Grows_per_s": n / t / 1e9, "Gnnz_per_s": mnz.numel() / t / 1e9}
+    out["note"] = "wall time of one call with a stream sync; bit-exact vs the oracle in tests/"
+    return out
+
+
+def cpu_spmat():
+    """The oracle's AGI loops at one PE on one host thread, smaller samples:
+    transpose / permute of a 2^22-row matrix of the same density, toposort at
+    2^20 rows."""
+    import numpy as np
+    import oracle
+    res = {}
+    n = 1 << 22
+    off, nz = oracle.gen_er(n, 18 / n, 0, 3, 101894)
+    t0 = time.perf_counter()
+    oracle.transpose_agi(n, n, off, nz)
+    t = time.perf_counter() - t0
+    res["transpose"] = {"rows": n, "nnz": int(nz.size), "Gnnz_per_s": nz.size / t / 1e9, "cores": 1, "kind": "port"}
+    rng = np.random.default_rng(1)
+    rpi, cpi = rng.permutation(n).astype(np.int64), rng.permutation(n).astype(np.int64)
+    t0 = time.perf_counter()
+    oracle.permute_agi(off, nz, rpi, cpi)
+    t = time.perf_counter() - t0
+    res["permute"] = {"rows": n, "nnz": int(nz.size), "Gnnz_per_s": nz.size / t / 1e9, "cores": 1, "kind": "port"}
+    n = 1 << 20
+    off, nz = oracle.gen_er(n, 70 / (n - 1), 1, 2, 5)
+    rpi, cpi = rng.permutation(n).astype(np.int64), rng.permutation(n).astype(np.int64)
+    moff, mnz = oracle.permute_agi(off, nz, rpi, cpi)
+    toff, tnz = oracle.transpose_agi(n, n, moff, mnz)
+    t0 = time.perf_counter()
+    oracle.toposort_agi(moff, mnz, toff, tnz)
+    t = time.perf_counter() - t0
+    res["toposort"] = {"rows": n, "nnz": int(mnz.size), "Grows_per_s": n / t / 1e9,
+                       "Gnnz_per_s": mnz.size / t / 1e9, "cores": 1, "kind": "port"}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,6 +502,12 @@ def main() -> int:
         if not args.no_cpu:
             extra["randperm"]["cpu_baseline"] = cpu_randperm()
         log(f"randperm: {extra['randperm']}")
+        log("sparse matrices: transpose, permute 2^24 rows; toposort 2^20 rows ...")
+        extra["spmat"] = bench_spmat(B, torch, dev, stream)
+        if not args.no_cpu:
+            extra["spmat"]["cpu_baseline"] = cpu_spmat()
+        log(f"spmat: {extra['spmat']}")
+        torch.cuda.empty_cache()
 
     if not args.no_batch:
         from bale_old_amd import selector as S

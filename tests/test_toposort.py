@@ -131,6 +131,18 @@ def test_toposort_matches_oracle(dev, orc, n, z, seed):
 
 
 @pytest.mark.gpu
+def test_toposort_separate_counters(dev, orc, monkeypatch):
+    """The path for matrices whose row counts and column sums do not share a
+    64-bit word, forced by BALE_TOPOSORT_SEPARATE."""
+    monkeypatch.setenv("BALE_TOPOSORT_SEPARATE", "1")
+    off, nz, toff, tnz = toposort_input(orc, 100000, 35, 5)
+    want_r, want_c, piv, lev = orc.toposort_agi(off, nz, toff, tnz)
+    rperm, cperm, levels = B.toposort(_dev(off), _dev(nz), _dev(toff), _dev(tnz))
+    assert levels == lev
+    assert np.array_equal(rperm.cpu().numpy(), want_r) and np.array_equal(cperm.cpu().numpy(), want_c)
+
+
+@pytest.mark.gpu
 def test_toposort_chain_has_one_level_per_row(dev, orc):
     n = 1000  # 1000 levels: many batches of launches
     rows = [[i, i + 1] if i + 1 < n else [i] for i in range(n)]
