@@ -64,7 +64,8 @@ static int convey_deliver(bale_convey* c, hipStream_t s) {
   const int npe = (int)c->npe;
   int nbits = 0;
   while ((1 << nbits) < npe) nbits++;
-  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus, ceil_div(n, kChunkAlign)));
+  // two chunks per CU (two delivery workgroups fit a CU's LDS)
+  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * 2, ceil_div(n, kChunkAlign)));
   const int64_t chunk = ceil_div(ceil_div(n, C), kChunkAlign) * kChunkAlign;
   C = ceil_div(n, chunk);
   void* ws = workspace(d, part_bytes(C, npe, 0, 1), s);

@@ -400,7 +400,9 @@ int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const 
   const int passes = (bits + 7) / 8;
   // chunks of the item array (one workgroup each, like the conveyor)
   const int64_t n = nnz;
-  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus, ceil_div(std::max<int64_t>(n, 1), kChunkAlign)));
+  // two chunks per CU: two delivery workgroups fit a CU's LDS, and the
+  // second hides the first one's barriers and loads
+  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * 2, ceil_div(std::max<int64_t>(n, 1), kChunkAlign)));
   const int64_t chunk = ceil_div(ceil_div(std::max<int64_t>(n, 1), C), kChunkAlign) * kChunkAlign;
   C = ceil_div(std::max<int64_t>(n, 1), chunk);
   const int Bmax = 256;
