@@ -65,9 +65,9 @@ def build_lib(jobs: int = 8) -> Path:
 
 
 def build_host() -> list[Path]:
-    """C drivers mirroring the reference's histo / ig programs."""
+    """C drivers mirroring the reference's programs."""
     out = []
-    for prog in ("histo", "ig"):
+    for prog in ("histo", "ig", "randperm", "transpose_matrix", "permute_matrix", "toposort"):
         src = HOST / f"{prog}.c"
         if not src.exists():
             continue
