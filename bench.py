@@ -433,8 +433,8 @@ def main() -> int:
     ap.add_argument("--batch-instances", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the multi-GPU exchange is a design (see DESIGN.md), "
-              "not a measured path; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the exchange between GPUs is designed, "
+              "not built or measured; run with --gpus 1", file=sys.stderr)
         return 2
 
     import torch
