@@ -85,6 +85,38 @@ def test_host_side_argument_checks():
     assert L.bale_probe_copy(1, 1, 15, None) == B.ERR_ARG
 
 
+def test_adjacent_kernels_check_arguments_on_the_host():
+    """randperm, transpose, permute, generator, toposort: sizes and modes are
+    refused before any device is touched (the pointers are never read)."""
+    L = B.lib()
+    assert L.bale_randperm(None, -1, 1, None) == B.ERR_ARG
+    assert L.bale_randperm(None, 0, 1, None) == B.OK
+    assert L.bale_randperm(1, 1 << 30, 1, None) == B.ERR_ARG and "2^30" in B.last_error()
+    assert L.bale_is_perm(None, 5, None) == B.ERR_ARG
+    assert L.rand_permp_agi(-1, 1) is None and "outside" in B.last_error()
+    assert L.rand_permp(1 << 30, 1) is None
+    big = 1 << 31
+    assert L.bale_spmat_transpose(-1, 4, 0, 1, 1, 1, 1, None) == B.ERR_ARG
+    assert L.bale_spmat_transpose(4, 4, big, 1, 1, 1, 1, None) == B.ERR_ARG and "2^31" in B.last_error()
+    assert L.bale_spmat_transpose(1 << 32, 4, 8, 1, 1, 1, 1, None) == B.ERR_ARG
+    assert L.bale_spmat_transpose(4, 4, 8, None, 1, 1, 1, None) == B.ERR_ARG
+    assert L.bale_spmat_permute(4, 4, 8, 1, 1, None, 1, 1, 1, None) == B.ERR_ARG
+    assert L.bale_spmat_permute(4, -4, 8, 1, 1, 1, 1, 1, 1, None) == B.ERR_ARG
+    nnz = C.c_int64(0)
+    assert L.bale_spmat_gen_er(10, 0.1, 0, 0, 1, 1, None, 0, C.byref(nnz), None) == B.ERR_ARG  # mode 0
+    assert L.bale_spmat_gen_er(10, 1.5, 0, 3, 1, 1, None, 0, C.byref(nnz), None) == B.ERR_ARG
+    assert L.bale_spmat_gen_er(-1, 0.1, 0, 3, 1, 1, None, 0, C.byref(nnz), None) == B.ERR_ARG
+    assert not L.gen_erdos_renyi_graph_dist(10, 0.1, 0, 0, 1)  # mode 0 is not provided
+    w, npiv = C.c_int64(0), C.c_int64(0)
+    assert L.bale_spmat_triangular(-1, 1, 1, 1, C.byref(w), C.byref(npiv), None) == B.ERR_ARG
+    assert L.bale_toposort(big, 1, 1, 1, 1, 1, 1, None, None) == B.ERR_ARG
+    assert L.bale_toposort(5, None, 1, 1, 1, 1, 1, None, None) == B.ERR_ARG
+    assert not L.transpose_matrix_agi(None) and "bad sparse matrix" in B.last_error()
+    m = B.SparseMat(0, 3, 3, 4, 0, 0, 1, 1, 0, 0)  # 3 x 4, no nonzeros: not square
+    assert L.toposort_matrix_agi(1, 1, C.byref(m), C.byref(m)) == -1.0
+    assert "square" in B.last_error()
+
+
 def test_dropin_rejects_negative_length():
     L = B.lib()
     assert L.histo_agi(None, -1, None) == -1.0

@@ -130,6 +130,74 @@ def test_gen_er_gaps_are_geometric(orc):
     assert chi2 < 30, (obs, exp)  # 7 degrees of freedom
 
 
+def py_det_log(x):
+    """The oracle's logarithm restated in Python floats, operation for
+    operation (Python floats are IEEE doubles and Python does not fuse)."""
+    import struct
+    b = struct.unpack("<Q", struct.pack("<d", x))[0]
+    e = ((b >> 52) & 0x7FF) - 1022
+    m = struct.unpack("<d", struct.pack("<Q", (b & 0x000FFFFFFFFFFFFF) | (0x3FE << 52)))[0]
+    if m < 0.70710678118654752440:
+        m = m * 2.0
+        e -= 1
+    s = (m - 1.0) / (m + 1.0)
+    z = s * s
+    t = 1.0 / 25.0
+    for k in range(11, -1, -1):
+        t = t * z + 1.0 / float(2 * k + 1)
+    return float(e) * 0.69314718055994530942 + 2.0 * s * t
+
+
+def py_gen_er(n, p, unit_diag, mode, seed):
+    """The generator restated in Python: per (row, half) a splitmix64 stream,
+    geometric gaps floor(log(u) / log(1 - p)), u uniform in (0, 1]."""
+    M = (1 << 64) - 1
+    G = 0x9E3779B97F4A7C15
+
+    def mix(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+
+    D = py_det_log(1.0 - p)
+
+    def half(row, h, lo, hi):
+        key = mix((seed + G * (2 * row + h + 1) + G) & M)
+        out, col, i = [], lo - 1, 0
+        while True:
+            x = mix((key + (i + 1) * G) & M)
+            i += 1
+            u = float((x >> 11) + 1) * 2.0 ** -53
+            g = math.floor(py_det_log(u) / D)
+            if g >= hi - col:
+                return out
+            col += 1 + g
+            if col >= hi:
+                return out
+            out.append(col)
+
+    rows = []
+    for r in range(n):
+        row = half(r, 0, 0, r) if mode in (1, 3) else []
+        if unit_diag:
+            row.append(r)
+        if mode in (2, 3):
+            row += half(r, 1, r + 1, n)
+        rows.append(row)
+    return make_csr(rows)
+
+
+@pytest.mark.parametrize("mode,unit_diag", [(1, 0), (2, 1), (3, 0), (3, 1)])
+def test_gen_er_matches_python_restatement(orc, mode, unit_diag):
+    n, p, seed = 300, 0.05, 1234567
+    off, nz = orc.gen_er(n, p, unit_diag, mode, seed)
+    woff, wnz = py_gen_er(n, p, unit_diag, mode, seed)
+    assert np.array_equal(off, woff) and np.array_equal(nz, wnz)
+    rng = np.random.default_rng(5)
+    for x in rng.random(2000) + 1e-12:
+        assert orc.det_log(float(x)) == py_det_log(float(x))
+
+
 def test_gen_er_gap_probabilities_exact(orc):
     # p = 1/2: a gap is 0 with probability 1/2 and 1 with probability 1/4 (a
     # shifted gap or a wrong logarithm base moves these by far more than the
