@@ -122,6 +122,33 @@ def test_randperm_matches_oracle(dev, orc, N, seed):
 
 
 @pytest.mark.gpu
+def test_randperm_large_matches_oracle(dev, orc):
+    """2^27 elements (2^28 slots, ~1.9e8 darts), element for element."""
+    import torch
+    N = 1 << 27
+    perm = torch.empty(N, dtype=torch.int64, device=dev)
+    B.randperm(perm, 101892)
+    want, _ = orc.randperm_agi(N, 101892)
+    assert np.array_equal(perm.cpu().numpy(), want)
+
+
+@pytest.mark.gpu
+def test_randperm_near_the_size_limit(dev):
+    """2^29 elements (2^30 slots, 7.4e8 darts): a permutation, and the same
+    one on a second call."""
+    import torch
+    N = 1 << 29
+    perm = torch.empty(N, dtype=torch.int64, device=dev)
+    B.randperm(perm, 7)
+    assert B.is_perm(perm)
+    head, tail = perm[:4096].clone(), perm[-4096:].clone()
+    B.randperm(perm, 7)
+    assert bool((perm[:4096] == head).all()) and bool((perm[-4096:] == tail).all())
+    del perm
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.gpu
 def test_randperm_seed0_uses_the_initial_state(dev, orc):
     import torch
     N = 5000
