@@ -324,7 +324,8 @@ def bench_randperm(B, torch, dev, stream, N=1 << 26, reps=5, seed=101892):
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
     t = median(ts)
-    return {"n": N, "seed": seed, "median_s": t, "Gelems_per_s": N / t / 1e9,
+    darts = int(1.3862944 * N + 8.0 * N ** 0.5) + 4096  # one device atomic min each
+    return {"n": N, "seed": seed, "median_s": t, "Gelems_per_s": N / t / 1e9, "darts": darts,
             "note": "wall time of one call (darts, first-dart table, hit ranks, placement, "
                     "compaction; one 4-byte readback of the hit count)"}
 
@@ -363,14 +364,18 @@ def bench_spmat(B, torch, dev, stream, reps=3):
     t, (toff, tnz) = timed(lambda: B.spmat_transpose(n, off, nz))
     ooff, onz = B.spmat_transpose(n, toff, tnz)
     assert bool((ooff == off).all()) and bool((onz == nz).all()), "transpose of the transpose differs"
-    out["transpose"] = {"rows": n, "nnz": nnz, "median_s": t, "Gnnz_per_s": nnz / t / 1e9}
+    # 16 algorithmic bytes per nonzero: A's column read, the transpose's row written
+    out["transpose"] = {"rows": n, "nnz": nnz, "median_s": t, "Gnnz_per_s": nnz / t / 1e9,
+                        "alg_GBps": 16 * nnz / t / 1e9}
     del toff, tnz, ooff, onz
     rpi = torch.empty(n, dtype=torch.int64, device=dev)
     cpi = torch.empty(n, dtype=torch.int64, device=dev)
     B.randperm(rpi, 1230)
     B.randperm(cpi, 45)
     t, _ = timed(lambda: B.spmat_permute(n, off, nz, rpi, cpi))
-    out["permute"] = {"rows": n, "nnz": nnz, "median_s": t, "Gnnz_per_s": nnz / t / 1e9}
+    # one random 8-byte read of cperminv per nonzero (a 128 MiB table)
+    out["permute"] = {"rows": n, "nnz": nnz, "median_s": t, "Gnnz_per_s": nnz / t / 1e9,
+                      "random_reads_G_per_s": nnz / t / 1e9}
     del off, nz, rpi, cpi
     torch.cuda.empty_cache()
     n = 1 << 20
@@ -384,8 +389,10 @@ def bench_spmat(B, torch, dev, stream, reps=3):
     t, (rp, cp, lev) = timed(lambda: B.toposort(moff, mnz, toff, tnz))
     po, pn = B.spmat_permute(n, moff, mnz, rp, cp)
     assert B.spmat_triangular(po, pn, True) == (0, 0), "toposort result is not triangular"
+    # two device atomics per nonzero (packed count and column sum, event key)
     out["toposort"] = {"rows": n, "nnz": mnz.numel(), "levels": lev, "median_s": t,
-                       "Grows_per_s": n / t / 1e9, "Gnnz_per_s": mnz.numel() / t / 1e9}
+                       "Grows_per_s": n / t / 1e9, "Gnnz_per_s": mnz.numel() / t / 1e9,
+                       "device_atomics_G_per_s": 2 * mnz.numel() / t / 1e9}
     out["note"] = "wall time of one call with a stream sync; bit-exact vs the oracle in tests/"
     return out
 
