@@ -304,6 +304,30 @@ def test_transpose_matches_oracle(dev, orc, numrows, numcols, mean, dup):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(8))
+def test_transpose_and_permute_random_shapes(dev, orc, seed):
+    """Seeded random shapes: rectangular, empty rows and columns, repeated
+    columns, a few long rows; transpose and permute against the oracle."""
+    rng = np.random.default_rng(1000 + seed)
+    numrows = int(rng.integers(1, 6000))
+    numcols = int(rng.choice([1, 2, 255, 256, 257, int(rng.integers(1, 1 << 17)), 1 << 20]))
+    lens = rng.poisson(float(rng.choice([0.3, 3.0, 12.0])), numrows)
+    lens[rng.integers(0, numrows, 3)] += int(rng.integers(0, 3000))  # a few long rows
+    off = np.zeros(numrows + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    nz = rng.integers(0, numcols, off[-1], dtype=np.int64)
+    toff, tnz = B.spmat_transpose(numcols, _dev(off), _dev(nz))
+    woff, wnz = orc.transpose_agi(numrows, numcols, off, nz)
+    assert np.array_equal(toff.cpu().numpy(), woff) and np.array_equal(tnz.cpu().numpy(), wnz)
+    # rectangular too: rperminv over the rows, cperminv over the columns
+    rpi = rng.permutation(numrows).astype(np.int64)
+    cpi = rng.permutation(numcols).astype(np.int64)
+    poff, pnz = B.spmat_permute(numcols, _dev(off), _dev(nz), _dev(rpi), _dev(cpi))
+    woff, wnz = orc.permute_agi(off, nz, rpi, cpi)
+    assert np.array_equal(poff.cpu().numpy(), woff) and np.array_equal(pnz.cpu().numpy(), wnz)
+
+
+@pytest.mark.gpu
 def test_transpose_er_and_back(dev, orc):
     n = 1 << 21
     off, nz = B.spmat_gen_er(n, 18 / n, 0, 3, seed=101894)

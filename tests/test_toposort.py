@@ -131,6 +131,23 @@ def test_toposort_matches_oracle(dev, orc, n, z, seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(6))
+def test_toposort_random_inputs(dev, orc, seed):
+    """Seeded random sizes and densities, from a handful of rows to 2^16, as
+    sparse as a chain and as dense as 200 per row."""
+    rng = np.random.default_rng(500 + seed)
+    n = int(rng.integers(2, 1 << 16))
+    z = float(rng.choice([0.5, 2.0, 10.0, 35.0, 200.0]))
+    off, nz, toff, tnz = toposort_input(orc, n, z, int(rng.integers(1, 1 << 30)),
+                                        int(rng.integers(1, 1 << 20)), int(rng.integers(1, 1 << 20)))
+    want_r, want_c, piv, lev = orc.toposort_agi(off, nz, toff, tnz)
+    assert piv == n
+    rperm, cperm, levels = B.toposort(_dev(off), _dev(nz), _dev(toff), _dev(tnz))
+    assert levels == lev
+    assert np.array_equal(rperm.cpu().numpy(), want_r) and np.array_equal(cperm.cpu().numpy(), want_c)
+
+
+@pytest.mark.gpu
 def test_toposort_separate_counters(dev, orc, monkeypatch):
     """The path for matrices whose row counts and column sums do not share a
     64-bit word, forced by BALE_TOPOSORT_SEPARATE."""
