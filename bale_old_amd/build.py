@@ -20,7 +20,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libbale_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["capi.hip", "spmat.hip", "toposort.hip", "gen.hip", "gather.hip", "histo.hip", "probe.hip", "batch.hip", "select.hip",
+HIP_SOURCES = ["capi.hip", "capi_spmat.hip", "spmat.hip", "toposort.hip", "gen.hip", "gather.hip", "histo.hip", "probe.hip", "batch.hip", "select.hip",
                "convey.hip", "randperm.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -92,6 +92,15 @@ Device* device_state();  // current device's state (created on first use)
 int bale_set_error(int code, const char* fmt, ...);
 // Workspace of at least `bytes`, 256-byte aligned; nullptr on failure.
 void* workspace(Device* d, size_t bytes, hipStream_t s);
+// BALE_ERR_HIP with the runtime's text for rc (bale_hip_last_error).
+int bale_fail_hip(int rc, const char* what);
+// The calling thread's bale_hip_last_timing values.
+void set_last_timing(double gpu_s, double h2d_s, double d2h_s);
+double last_gpu_seconds();
+// Whether p is device (or managed) memory.
+bool is_device_ptr(const void* p);
+// Seconds between two recorded events.
+double elapsed(hipEvent_t a, hipEvent_t b);
 
 // ---- launchers (return hipError_t-like int, 0 = ok) -----------------------
 int launch_fill_splitmix(int64_t* t, int64_t n, hipStream_t s);

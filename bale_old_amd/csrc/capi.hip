@@ -1,5 +1,6 @@
 // capi.hip -- the C-ABI of libbale_hip.so: per-device state, the native
-// bale_* API and the reference drop-ins (histo_*, ig_*).
+// bale_* API and the reference drop-ins (histo_*, ig_*).  The random
+// permutation, sparse-matrix and toposort entry points are in capi_spmat.hip.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -38,6 +39,13 @@ int bale_set_error(int code, const char* fmt, ...) {
 static int fail_hip(int rc, const char* what) {
   return fail(BALE_ERR_HIP, "%s: %s", what, hipGetErrorString((hipError_t)rc));
 }
+int bale_fail_hip(int rc, const char* what) { return fail_hip(rc, what); }
+void set_last_timing(double gpu_s, double h2d_s, double d2h_s) {
+  t_gpu_s = gpu_s;
+  t_h2d_s = h2d_s;
+  t_d2h_s = d2h_s;
+}
+double last_gpu_seconds() { return t_gpu_s; }
 
 constexpr int kMaxDevices = 64;
 static Device g_dev[kMaxDevices];
@@ -127,7 +135,7 @@ static void plan_gather(const Device*, int64_t, int64_t, bale_config* c) {
   c->merge = BALE_MERGE_NONE;
 }
 
-static bool is_device_ptr(const void* p) {
+bool is_device_ptr(const void* p) {
   hipPointerAttribute_t a;
   memset(&a, 0, sizeof a);
   hipError_t rc = hipPointerGetAttributes(&a, p);
@@ -399,7 +407,7 @@ struct Staged {
   }
 };
 
-static double elapsed(hipEvent_t a, hipEvent_t b) {
+double elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
   return ms * 1e-3;
@@ -579,442 +587,6 @@ double ig_exstack_function(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, in
 }
 double ig_exstack_pkg(int64_t* tgt, int64_t* pckindx, int64_t l_num_req, int64_t* ltable, int64_t) {
   return dropin(1, tgt, pckindx, l_num_req, ltable, true, BALE_STRAT_BUCKET, BALE_MERGE_NONE);
-}
-
-}  // extern "C"
-
-// ======================= random permutation ================================
-extern "C" {
-
-int bale_randperm(int64_t* perm, int64_t n, int64_t seed, void* stream) {
-  if (n < 0) return fail(BALE_ERR_ARG, "negative permutation length %lld", (long long)n);
-  if (n == 0) return BALE_OK;
-  if (!perm) return fail(BALE_ERR_ARG, "null permutation");
-  if (n >= ((int64_t)1 << 30))
-    return fail(BALE_ERR_ARG, "permutation length must be below 2^30 (slots 2n below 2^31)");
-  Device* d = device_state();
-  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  int rc = launch_randperm(perm, n, seed, d, (hipStream_t)stream, nullptr);
-  if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "workspace allocation failed");
-  return rc ? fail_hip(rc, "randperm") : BALE_OK;
-}
-
-int bale_is_perm(const int64_t* perm, int64_t n, void* stream) {
-  if (n < 0 || (n > 0 && !perm)) return fail(BALE_ERR_ARG, "bad permutation array");
-  Device* d = device_state();
-  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  int ok = 0;
-  int rc = launch_check_perm(perm, n, d, (hipStream_t)stream, &ok);
-  if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "workspace allocation failed");
-  if (rc) return fail_hip(rc, "is_perm");
-  return ok;
-}
-
-// The reference's models of rand_permp (spmat/spmat.h:115-118) all return a
-// permutation in the shared heap; here it is device memory from
-// bale_all_alloc, filled by the dart-throwing AGI model (the result the
-// reference's rand_permp_agi gives at one PE), and the GPU seconds go to
-// bale_hip_last_timing.  NULL on error.
-static int64_t* randperm_dropin(int64_t N, int seed) {
-  t_gpu_s = t_h2d_s = t_d2h_s = 0.0;
-  if (N < 0 || N >= ((int64_t)1 << 30)) {
-    fail(BALE_ERR_ARG, "permutation length %lld outside [0, 2^30)", (long long)N);
-    return nullptr;
-  }
-  int64_t* perm = (int64_t*)bale_all_alloc(N ? N : 1, sizeof(int64_t));
-  if (!perm) return nullptr;
-  Device* d = device_state();
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventRecord(e0, 0);
-  int rc = N ? launch_randperm(perm, N, seed, d, 0, nullptr) : 0;
-  hipEventRecord(e1, 0);
-  if (!rc) rc = (int)hipEventSynchronize(e1);
-  if (!rc) t_gpu_s = elapsed(e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  if (rc) {
-    bale_all_free(perm);
-    if (rc == hipErrorOutOfMemory) fail(BALE_ERR_ALLOC, "workspace allocation failed");
-    else fail_hip(rc, "randperm");
-    return nullptr;
-  }
-  return perm;
-}
-
-int64_t* rand_permp_agi(int64_t N, int seed) { return randperm_dropin(N, seed); }
-int64_t* rand_permp_exstack(int64_t N, int seed, int64_t) { return randperm_dropin(N, seed); }
-int64_t* rand_permp_exstack2(int64_t N, int seed, int64_t) { return randperm_dropin(N, seed); }
-int64_t* rand_permp_conveyor(int64_t N, int seed) { return randperm_dropin(N, seed); }
-
-// spmat/spmat.h:152 -- 1 if perm (device memory) holds 0..N-1 once each.
-int is_perm(int64_t* perm, int64_t N) {
-  int r = bale_is_perm(perm, N, nullptr);
-  return r > 0 ? 1 : 0;
-}
-
-int64_t* rand_permp(int64_t N, int seed) { return randperm_dropin(N, seed); }
-
-// ---- sparse matrices: native API ------------------------------------------------
-
-static int spmat_rc(int rc, const char* what) {
-  if (rc < 0) return rc;  // a BALE_ERR_* code, text already set
-  if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "%s: workspace allocation failed", what);
-  return fail_hip(rc, what);
-}
-
-int bale_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const int64_t* offset,
-                         const int64_t* nonzero, int64_t* t_offset, int64_t* t_nonzero,
-                         void* stream) {
-  if (numrows < 0 || numcols < 0 || nnz < 0) return fail(BALE_ERR_ARG, "negative matrix size");
-  if (numrows >= ((int64_t)1 << 32) || numcols >= ((int64_t)1 << 32) || nnz >= ((int64_t)1 << 31))
-    return fail(BALE_ERR_ARG, "transpose needs numrows, numcols < 2^32 and nnz < 2^31");
-  if (!offset || !t_offset || (nnz && (!nonzero || !t_nonzero)))
-    return fail(BALE_ERR_ARG, "null matrix array");
-  Device* d = device_state();
-  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  int rc = launch_spmat_transpose(numrows, numcols, nnz, offset, nonzero, t_offset, t_nonzero, d,
-                                  (hipStream_t)stream);
-  if (rc) return spmat_rc(rc, "transpose");
-  return bale_sync_errors(stream, nullptr);
-}
-
-int bale_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const int64_t* offset,
-                       const int64_t* nonzero, const int64_t* rperminv, const int64_t* cperminv,
-                       int64_t* p_offset, int64_t* p_nonzero, void* stream) {
-  if (numrows < 0 || numcols < 0 || nnz < 0) return fail(BALE_ERR_ARG, "negative matrix size");
-  if (!offset || !p_offset || (numrows && !rperminv) || (nnz && (!nonzero || !p_nonzero || !cperminv)))
-    return fail(BALE_ERR_ARG, "null matrix or permutation array");
-  Device* d = device_state();
-  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  int rc = launch_spmat_permute(numrows, numcols, nnz, offset, nonzero, rperminv, cperminv, p_offset,
-                                p_nonzero, d, (hipStream_t)stream);
-  if (rc) return spmat_rc(rc, "permute");
-  return bale_sync_errors(stream, nullptr);
-}
-
-int bale_spmat_gen_er(int64_t n, double p, int unit_diag, int mode, uint64_t seed, int64_t* offset,
-                      int64_t* nonzero, int64_t nnz_cap, int64_t* nnz, void* stream) {
-  if (n < 0 || n >= ((int64_t)1 << 32)) return fail(BALE_ERR_ARG, "row count outside [0, 2^32)");
-  if (mode < 1 || mode > 3) return fail(BALE_ERR_ARG, "mode %d (1 lower, 2 upper, 3 both)", mode);
-  if (!(p >= 0.0 && p <= 1.0)) return fail(BALE_ERR_ARG, "edge probability outside [0, 1]");
-  if (!offset || !nnz) return fail(BALE_ERR_ARG, "null offset or nnz");
-  Device* d = device_state();
-  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  hipStream_t s = (hipStream_t)stream;
-  int rc = launch_spmat_gen_er(n, p, unit_diag, mode, seed, offset, nullptr, d, s);
-  if (rc) return spmat_rc(rc, "gen_er");
-  int64_t total = 0;
-  rc = (int)hipMemcpyAsync(&total, offset + n, 8, hipMemcpyDeviceToHost, s);
-  if (!rc) rc = (int)hipStreamSynchronize(s);
-  if (rc) return fail_hip(rc, "gen_er: reading nnz");
-  *nnz = total;
-  if (!nonzero) return BALE_OK;
-  if (total > nnz_cap)
-    return fail(BALE_ERR_ARG, "gen_er: %lld nonzeros, capacity %lld", (long long)total, (long long)nnz_cap);
-  rc = launch_spmat_gen_er(n, p, unit_diag, mode, seed, offset, nonzero, d, s);
-  if (!rc) rc = (int)hipStreamSynchronize(s);
-  return rc ? spmat_rc(rc, "gen_er") : BALE_OK;
-}
-
-int bale_toposort(int64_t n, const int64_t* offset, const int64_t* nonzero, const int64_t* t_offset,
-                  const int64_t* t_nonzero, int64_t* rperm, int64_t* cperm, int64_t* levels,
-                  void* stream) {
-  if (n < 0 || n >= ((int64_t)1 << 31)) return fail(BALE_ERR_ARG, "toposort: n outside [0, 2^31)");
-  if (n && (!offset || !nonzero || !t_offset || !t_nonzero || !rperm || !cperm))
-    return fail(BALE_ERR_ARG, "toposort: null array");
-  Device* d = device_state();
-  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  int64_t piv = 0, lev = 0, bad_col = 0, twice = 0;
-  int rc = launch_toposort(n, offset, nonzero, t_offset, t_nonzero, rperm, cperm, d,
-                           (hipStream_t)stream, &piv, &lev, &bad_col, &twice);
-  if (rc) return spmat_rc(rc, "toposort");
-  if (levels) *levels = lev;
-  if (bad_col) return fail(BALE_ERR_RANGE, "toposort: %lld column indices outside [0, %lld)",
-                           (long long)bad_col, (long long)n);
-  if (twice || piv != n)
-    return fail(BALE_ERR_ARG, "toposort: %lld pivots for %lld rows, %lld columns claimed twice: "
-                "the matrix is not morally upper triangular", (long long)piv, (long long)n,
-                (long long)twice);
-  return BALE_OK;
-}
-
-int bale_spmat_triangular(int64_t numrows, const int64_t* offset, const int64_t* nonzero, int upper,
-                          int64_t* wrong, int64_t* nopivot, void* stream) {
-  if (numrows < 0 || !offset || !wrong || !nopivot) return fail(BALE_ERR_ARG, "bad arguments");
-  Device* d = device_state();
-  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  int rc = launch_spmat_triangular(offset, nonzero, numrows, upper, d, (hipStream_t)stream, wrong,
-                                   nopivot);
-  return rc ? spmat_rc(rc, "triangular") : BALE_OK;
-}
-
-}  // extern "C"
-
-// ---- sparse matrices: the reference's spmat interface ----------------------------
-namespace bale {
-
-// A device copy of n int64 values when p is host memory (freed with the
-// object), else p itself.
-struct StagedI64 {
-  const int64_t* p = nullptr;
-  int64_t* owned = nullptr;
-  ~StagedI64() {
-    if (owned) (void)hipFree(owned);
-  }
-  int stage(const int64_t* src, int64_t n) {
-    if (!src || n <= 0 || is_device_ptr(src)) {
-      p = src;
-      return 0;
-    }
-    if (hipMalloc((void**)&owned, (size_t)n * 8) != hipSuccess) {
-      (void)hipGetLastError();
-      owned = nullptr;
-      return fail(BALE_ERR_ALLOC, "staging %lld values", (long long)n);
-    }
-    int rc = (int)hipMemcpy(owned, src, (size_t)n * 8, hipMemcpyHostToDevice);
-    if (rc) return fail_hip(rc, "staging to the device");
-    p = owned;
-    return 0;
-  }
-};
-
-static sparsemat_t* new_matrix(int64_t numrows, int64_t numcols, int64_t nnz) {
-  sparsemat_t* m = (sparsemat_t*)calloc(1, sizeof(sparsemat_t));
-  if (!m) {
-    fail(BALE_ERR_ALLOC, "matrix struct");
-    return nullptr;
-  }
-  m->numrows = m->lnumrows = numrows;
-  m->numcols = numcols;
-  m->nnz = m->lnnz = nnz;
-  m->offset = m->loffset = (int64_t*)bale_all_alloc(numrows + 1, sizeof(int64_t));
-  m->nonzero = m->lnonzero = (int64_t*)bale_all_alloc(nnz ? nnz : 1, sizeof(int64_t));
-  if (!m->offset || !m->nonzero) {
-    bale_all_free(m->offset);
-    bale_all_free(m->nonzero);
-    free(m);
-    return nullptr;
-  }
-  return m;
-}
-
-static void drop_matrix(sparsemat_t* m) {
-  if (!m) return;
-  bale_all_free(m->offset);
-  bale_all_free(m->nonzero);
-  free(m);
-}
-
-static bool matrix_ok(const sparsemat_t* A) {
-  if (!A || A->numrows < 0 || A->numcols < 0 || A->nnz < 0 || !A->offset ||
-      (A->nnz && !A->nonzero)) {
-    fail(BALE_ERR_ARG, "bad sparse matrix");
-    return false;
-  }
-  return true;
-}
-
-// Times the GPU part of a spmat drop-in on the null stream.
-struct DropinClock {
-  hipEvent_t e0{}, e1{};
-  DropinClock() {
-    t_gpu_s = t_h2d_s = t_d2h_s = 0.0;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, 0);
-  }
-  void stop() {
-    hipEventRecord(e1, 0);
-    if (hipEventSynchronize(e1) == hipSuccess) t_gpu_s = elapsed(e0, e1);
-  }
-  ~DropinClock() {
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-  }
-};
-
-static sparsemat_t* transpose_dropin(sparsemat_t* A) {
-  if (!matrix_ok(A)) return nullptr;
-  StagedI64 off, nz;
-  if (off.stage(A->offset, A->numrows + 1) || nz.stage(A->nonzero, A->nnz)) return nullptr;
-  sparsemat_t* At = new_matrix(A->numcols, A->numrows, A->nnz);
-  if (!At) return nullptr;
-  DropinClock clk;
-  int rc = bale_spmat_transpose(A->numrows, A->numcols, A->nnz, off.p, nz.p, At->offset, At->nonzero,
-                                nullptr);
-  clk.stop();
-  if (rc) {
-    drop_matrix(At);
-    return nullptr;
-  }
-  return At;
-}
-
-static sparsemat_t* permute_dropin(sparsemat_t* A, const int64_t* rperminv, const int64_t* cperminv) {
-  if (!matrix_ok(A)) return nullptr;
-  StagedI64 off, nz, rp, cp;
-  if (off.stage(A->offset, A->numrows + 1) || nz.stage(A->nonzero, A->nnz) ||
-      rp.stage(rperminv, A->numrows) || cp.stage(cperminv, A->numcols))
-    return nullptr;
-  sparsemat_t* Ap = new_matrix(A->numrows, A->numcols, A->nnz);
-  if (!Ap) return nullptr;
-  DropinClock clk;
-  int rc = bale_spmat_permute(A->numrows, A->numcols, A->nnz, off.p, nz.p, rp.p, cp.p, Ap->offset,
-                              Ap->nonzero, nullptr);
-  clk.stop();
-  if (rc) {
-    drop_matrix(Ap);
-    return nullptr;
-  }
-  return Ap;
-}
-
-static double toposort_dropin(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
-  if (!matrix_ok(mat) || !matrix_ok(tmat)) return -1.0;
-  if (mat->numrows != mat->numcols || tmat->numrows != mat->numcols || tmat->nnz != mat->nnz) {
-    fail(BALE_ERR_ARG, "toposort: mat must be square and tmat its transpose");
-    return -1.0;
-  }
-  StagedI64 off, nz, toff, tnz;
-  if (off.stage(mat->offset, mat->numrows + 1) || nz.stage(mat->nonzero, mat->nnz) ||
-      toff.stage(tmat->offset, tmat->numrows + 1) || tnz.stage(tmat->nonzero, tmat->nnz))
-    return -1.0;
-  DropinClock clk;
-  int rc = bale_toposort(mat->numrows, off.p, nz.p, toff.p, tnz.p, rperm, cperm, nullptr, nullptr);
-  clk.stop();
-  return rc ? -1.0 : t_gpu_s;
-}
-
-static int triangular_dropin(sparsemat_t* A, int64_t unit_diagonal, int upper) {
-  if (!matrix_ok(A)) return 0;
-  StagedI64 off, nz;
-  if (off.stage(A->offset, A->numrows + 1) || nz.stage(A->nonzero, A->nnz)) return 0;
-  int64_t wrong = 0, nopivot = 0;
-  if (bale_spmat_triangular(A->numrows, off.p, nz.p, upper, &wrong, &nopivot, nullptr)) return 0;
-  return !(wrong || (unit_diagonal && nopivot));
-}
-
-}  // namespace bale
-
-extern "C" {
-
-sparsemat_t* transpose_matrix_agi(sparsemat_t* A) { return transpose_dropin(A); }
-sparsemat_t* transpose_matrix_exstack(sparsemat_t* A, int64_t) { return transpose_dropin(A); }
-sparsemat_t* transpose_matrix_exstack2(sparsemat_t* A, int64_t) { return transpose_dropin(A); }
-sparsemat_t* transpose_matrix_conveyor(sparsemat_t* A) { return transpose_dropin(A); }
-sparsemat_t* transpose_matrix(sparsemat_t* A) { return transpose_dropin(A); }
-
-sparsemat_t* permute_matrix_agi(sparsemat_t* A, int64_t* rperminv, int64_t* cperminv) {
-  return permute_dropin(A, rperminv, cperminv);
-}
-sparsemat_t* permute_matrix_exstack(sparsemat_t* A, int64_t* rperminv, int64_t* cperminv, int64_t) {
-  return permute_dropin(A, rperminv, cperminv);
-}
-sparsemat_t* permute_matrix_exstack2(sparsemat_t* A, int64_t* rperminv, int64_t* cperminv, int64_t) {
-  return permute_dropin(A, rperminv, cperminv);
-}
-sparsemat_t* permute_matrix_conveyor(sparsemat_t* A, int64_t* rperminv, int64_t* cperminv) {
-  return permute_dropin(A, rperminv, cperminv);
-}
-sparsemat_t* permute_matrix(sparsemat_t* A, int64_t* rperminv, int64_t* cperminv) {
-  return permute_dropin(A, rperminv, cperminv);
-}
-
-sparsemat_t* gen_erdos_renyi_graph_dist(int n, double p, int64_t unit_diag, int64_t mode, int64_t seed) {
-  t_gpu_s = t_h2d_s = t_d2h_s = 0.0;
-  if (n < 0 || mode < 1 || mode > 3) {
-    fail(BALE_ERR_ARG, "gen_erdos_renyi_graph_dist: n %d, mode %lld (1, 2 or 3)", n, (long long)mode);
-    return nullptr;
-  }
-  int64_t* off = (int64_t*)bale_all_alloc((int64_t)n + 1, sizeof(int64_t));
-  if (!off) return nullptr;
-  int64_t nnz = 0;
-  if (bale_spmat_gen_er(n, p, (int)(unit_diag != 0), (int)mode, (uint64_t)seed, off, nullptr, 0, &nnz,
-                        nullptr)) {
-    bale_all_free(off);
-    return nullptr;
-  }
-  sparsemat_t* m = (sparsemat_t*)calloc(1, sizeof(sparsemat_t));
-  int64_t* nz = (int64_t*)bale_all_alloc(nnz ? nnz : 1, sizeof(int64_t));
-  if (!m || !nz ||
-      bale_spmat_gen_er(n, p, (int)(unit_diag != 0), (int)mode, (uint64_t)seed, off, nz, nnz, &nnz,
-                        nullptr)) {
-    free(m);
-    bale_all_free(nz);
-    bale_all_free(off);
-    if (!m) fail(BALE_ERR_ALLOC, "matrix struct");
-    return nullptr;
-  }
-  m->numrows = m->lnumrows = n;
-  m->numcols = n;
-  m->nnz = m->lnnz = nnz;
-  m->offset = m->loffset = off;
-  m->nonzero = m->lnonzero = nz;
-  return m;
-}
-
-sparsemat_t* init_matrix(int64_t numrows, int64_t numcols, int64_t nnz) {
-  if (numrows < 0 || numcols < 0 || nnz < 0) {
-    fail(BALE_ERR_ARG, "init_matrix: negative size");
-    return nullptr;
-  }
-  return new_matrix(numrows, numcols, nnz);
-}
-
-void clear_matrix(sparsemat_t* mat) {
-  if (!mat) return;
-  bale_all_free(mat->offset);
-  bale_all_free(mat->nonzero);
-  mat->offset = mat->loffset = mat->nonzero = mat->lnonzero = nullptr;
-}
-
-int compare_matrix(sparsemat_t* lmat, sparsemat_t* rmat) {
-  if (!matrix_ok(lmat) || !matrix_ok(rmat)) return 1;
-  if (lmat->numrows != rmat->numrows || lmat->numcols != rmat->numcols || lmat->nnz != rmat->nnz)
-    return 1;
-  Device* d = device_state();
-  if (!d) return 1;
-  StagedI64 lo, ro, ln, rn;
-  if (lo.stage(lmat->offset, lmat->numrows + 1) || ro.stage(rmat->offset, rmat->numrows + 1) ||
-      ln.stage(lmat->nonzero, lmat->nnz) || rn.stage(rmat->nonzero, rmat->nnz))
-    return 1;
-  int64_t d1 = 0, d2 = 0;
-  if (launch_count_diff(lo.p, ro.p, lmat->numrows + 1, d, 0, &d1)) return 1;
-  if (d1) return 1;
-  if (launch_count_diff(ln.p, rn.p, lmat->nnz, d, 0, &d2)) return 1;
-  return d2 ? 1 : 0;
-}
-
-double toposort_matrix_agi(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
-  return toposort_dropin(rperm, cperm, mat, tmat);
-}
-double toposort_matrix_exstack(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
-  return toposort_dropin(rperm, cperm, mat, tmat);
-}
-double toposort_matrix_exstack2(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
-  return toposort_dropin(rperm, cperm, mat, tmat);
-}
-double toposort_matrix_convey(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
-  return toposort_dropin(rperm, cperm, mat, tmat);
-}
-double toposort_matrix_oo(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
-  return toposort_dropin(rperm, cperm, mat, tmat);
-}
-double toposort_matrix_cooler(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, sparsemat_t* tmat) {
-  return toposort_dropin(rperm, cperm, mat, tmat);
-}
-double toposort_matrix_exstack_orig(int64_t* rperm, int64_t* cperm, sparsemat_t* mat,
-                                    sparsemat_t* tmat) {
-  return toposort_dropin(rperm, cperm, mat, tmat);
-}
-
-int is_upper_triangular(sparsemat_t* A, int64_t unit_diagonal) {
-  return triangular_dropin(A, unit_diagonal, 1);
-}
-int is_lower_triangular(sparsemat_t* A, int64_t unit_diagonal) {
-  return triangular_dropin(A, unit_diagonal, 0);
 }
 
 }  // extern "C"
