@@ -74,12 +74,6 @@ int64_t* rand_permp_exstack(int64_t N, int seed, int64_t) { return randperm_drop
 int64_t* rand_permp_exstack2(int64_t N, int seed, int64_t) { return randperm_dropin(N, seed); }
 int64_t* rand_permp_conveyor(int64_t N, int seed) { return randperm_dropin(N, seed); }
 
-// spmat/spmat.h:152 -- 1 if perm (device memory) holds 0..N-1 once each.
-int is_perm(int64_t* perm, int64_t N) {
-  int r = bale_is_perm(perm, N, nullptr);
-  return r > 0 ? 1 : 0;
-}
-
 int64_t* rand_permp(int64_t N, int seed) { return randperm_dropin(N, seed); }
 
 // ---- sparse matrices: native API ------------------------------------------------
@@ -206,6 +200,34 @@ struct StagedI64 {
   }
 };
 
+// Device memory for an output of n int64 values: `dst` itself when it is
+// device memory, else a device buffer copied back to `dst` by back().
+struct StagedOut {
+  int64_t* p = nullptr;
+  int64_t* owned = nullptr;
+  ~StagedOut() {
+    if (owned) (void)hipFree(owned);
+  }
+  int stage(int64_t* dst, int64_t n) {
+    if (n <= 0 || is_device_ptr(dst)) {
+      p = dst;
+      return 0;
+    }
+    if (hipMalloc((void**)&owned, (size_t)n * 8) != hipSuccess) {
+      (void)hipGetLastError();
+      owned = nullptr;
+      return bale_set_error(BALE_ERR_ALLOC, "staging %lld values", (long long)n);
+    }
+    p = owned;
+    return 0;
+  }
+  int back(int64_t* dst, int64_t n) {
+    if (!owned) return 0;
+    int rc = (int)hipMemcpy(dst, owned, (size_t)n * 8, hipMemcpyDeviceToHost);
+    return rc ? bale_fail_hip(rc, "copying back to the host") : 0;
+  }
+};
+
 static sparsemat_t* new_matrix(int64_t numrows, int64_t numcols, int64_t nnz) {
   sparsemat_t* m = (sparsemat_t*)calloc(1, sizeof(sparsemat_t));
   if (!m) {
@@ -307,10 +329,26 @@ static double toposort_dropin(int64_t* rperm, int64_t* cperm, sparsemat_t* mat, 
   if (off.stage(mat->offset, mat->numrows + 1) || nz.stage(mat->nonzero, mat->nnz) ||
       toff.stage(tmat->offset, tmat->numrows + 1) || tnz.stage(tmat->nonzero, tmat->nnz))
     return -1.0;
+  const int64_t n = mat->numrows;
+  if (!rperm || !cperm) {
+    bale_set_error(BALE_ERR_ARG, "toposort: null permutation");
+    return -1.0;
+  }
+  // host arrays for the permutations get device copies, written back after
+  StagedOut rp, cp;
+  if (rp.stage(rperm, n) || cp.stage(cperm, n)) return -1.0;
   DropinClock clk;
-  int rc = bale_toposort(mat->numrows, off.p, nz.p, toff.p, tnz.p, rperm, cperm, nullptr, nullptr);
+  int rc = bale_toposort(n, off.p, nz.p, toff.p, tnz.p, rp.p, cp.p, nullptr, nullptr);
   clk.stop();
+  if (!rc) rc = rp.back(rperm, n);
+  if (!rc) rc = cp.back(cperm, n);
   return rc ? -1.0 : last_gpu_seconds();
+}
+
+static int is_perm_dropin(const int64_t* perm, int64_t N) {
+  StagedI64 p;
+  if (N < 0 || p.stage(perm, N)) return 0;
+  return bale_is_perm(p.p, N, nullptr) > 0 ? 1 : 0;
 }
 
 static int triangular_dropin(sparsemat_t* A, int64_t unit_diagonal, int upper) {
@@ -435,6 +473,10 @@ double toposort_matrix_exstack_orig(int64_t* rperm, int64_t* cperm, sparsemat_t*
                                     sparsemat_t* tmat) {
   return toposort_dropin(rperm, cperm, mat, tmat);
 }
+
+// spmat/spmat.h:152 -- 1 if perm holds 0..N-1 once each (checked on the GPU;
+// host arrays are staged).
+int is_perm(int64_t* perm, int64_t N) { return is_perm_dropin(perm, N); }
 
 int is_upper_triangular(sparsemat_t* A, int64_t unit_diagonal) {
   return triangular_dropin(A, unit_diagonal, 1);

@@ -82,7 +82,7 @@ int64_t *rand_permp_agi(int64_t N, int seed);
 int64_t *rand_permp_exstack(int64_t N, int seed, int64_t buf_cnt);
 int64_t *rand_permp_exstack2(int64_t N, int seed, int64_t buf_cnt);
 int64_t *rand_permp_conveyor(int64_t N, int seed);
-/* spmat/spmat.h:152 -- 1 if perm (device memory) holds 0..N-1 exactly once. */
+/* spmat/spmat.h:152 -- 1 if perm (device or host memory) holds 0..N-1 exactly once. */
 int is_perm(int64_t *perm, int64_t N);
 /* spmat/spmat_utils.upc:92-107 -- the reference's wrapper (its default model, AGI). */
 int64_t *rand_permp(int64_t N, int seed);
@@ -152,7 +152,7 @@ int is_lower_triangular(sparsemat_t *A, int64_t unit_diagonal);
 
 /* apps/topo_src/toposort.h:47-50 and alternates/toposort_alternates.h:42-44
  * (apps/topo_src/toposort_agi.upc:53-166) -- rperm[row], cperm[col] (device
- * memory, numrows entries each) such that permute_matrix(mat, rperm, cperm)
+ * or host memory, numrows entries each) such that permute_matrix(mat, rperm, cperm)
  * is upper triangular with a full diagonal; mat is a row and column
  * permutation of such a matrix and tmat its transpose.  Every model returns
  * the permutations toposort_matrix_agi finds at one PE (the other models

@@ -176,6 +176,11 @@ def test_randperm_dropins_and_is_perm(dev, orc):
         L.bale_all_free(p)
     assert L.rand_permp_agi(-1, 1) is None
     assert "outside" in B.last_error()
+    # is_perm on host arrays (staged to the GPU)
+    assert L.is_perm(want.ctypes.data, N) == 1
+    bad = want.copy()
+    bad[17] = bad[18]
+    assert L.is_perm(bad.ctypes.data, N) == 0
 
 
 @pytest.mark.gpu

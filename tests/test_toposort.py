@@ -228,6 +228,11 @@ def test_toposort_dropins_on_the_driver_pipeline(dev, orc):
         mat2 = L.permute_matrix(mat, rperm, cperm)
         assert L.is_upper_triangular(mat2, 1) == 1
         L.clear_matrix(mat2)
+    # host arrays for the permutations work too (staged and copied back)
+    hr, hc = np.full(n, -7, np.int64), np.full(n, -7, np.int64)
+    assert L.toposort_matrix_agi(hr.ctypes.data, hc.ctypes.data, mat, tmat) > 0, B.last_error()
+    assert np.array_equal(hr, want_r) and np.array_equal(hc, want_c)
+    assert L.is_perm(hr.ctypes.data, n) == 1
     for x in (omat, mat, tmat):
         L.clear_matrix(x)
     for ptr in (rpi, cpi, rperm, cperm):
