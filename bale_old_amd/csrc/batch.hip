@@ -820,18 +820,15 @@ int bale_tune(bale_instance* inst, int64_t count, int64_t min_ops, float* costs,
     int64_t R = std::max<int64_t>(1, std::min<int64_t>(256, ceil_div(min_ops, in.n)));
     float best = 3.4e38f;
     int bestk = 0;
-    for (int k = 0; k < nc; k++) {
-      bale_instance t = in;
-      t.cfg = cand[k];
-      if (in.kind == 1) t.tgt = (int64_t*)scratch.p;
-      else t.table = scratch.p;
+    // Seconds of one run of `copies` copies of candidate t as a batch (after a
+    // warm-up run); < 0 when the plan does not take it, -2 on a failed run.
+    auto time_copies = [&](const bale_instance& t, int64_t copies) -> float {
       Plan P;
-      bool ok = true;
-      for (int64_t r = 0; r < R && ok; r++) ok = plan_add(P, t, d) == BALE_OK;
-      if (!ok) continue;
+      for (int64_t r = 0; r < copies; r++)
+        if (plan_add(P, t, d) != BALE_OK) return -1.f;
       if (plan_upload(P, bufs, s)) {
         rc = bale_set_error(BALE_ERR_ALLOC, "tuner upload");
-        break;
+        return -2.f;
       }
       set_batch_lds(P);
       plan_enqueue(P, bufs, d, s);  // warm-up
@@ -840,11 +837,29 @@ int bale_tune(bale_instance* inst, int64_t count, int64_t min_ops, float* costs,
       (void)hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) {
         rc = bale_set_error(BALE_ERR_HIP, "tuner run failed");
-        break;
+        return -2.f;
       }
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, e0, e1);
-      float cost = ms * 1e-3f / (float)(R * in.n);
+      return ms * 1e-3f;
+    };
+    for (int k = 0; k < nc; k++) {
+      bale_instance t = in;
+      t.cfg = cand[k];
+      if (in.kind == 1) t.tgt = (int64_t*)scratch.p;
+      else t.table = scratch.p;
+      // The marginal cost: in a large batch a strategy's phases run anyway
+      // (launches, sketches, the bucketed plan's scans), so the cost an
+      // instance adds is the difference between R and 2R copies, not R copies
+      // alone; floored at a tenth of the average so noise cannot make it 0.
+      const float t1 = time_copies(t, R);
+      if (t1 == -2.f) break;
+      if (t1 < 0.f) continue;
+      const float t2 = time_copies(t, 2 * R);
+      if (t2 == -2.f) break;
+      if (t2 < 0.f) continue;
+      const float avg = t2 / (float)(2 * R * in.n);
+      const float cost = std::max((t2 - t1) / (float)(R * in.n), 0.1f * avg);
       if (costs && k < max_cand) costs[i * max_cand + k] = cost;
       if (cost < best) {
         best = cost;

@@ -329,8 +329,10 @@ void bale_batch_destroy(bale_batch *b);
 int bale_batch_launch_plain(const bale_instance *inst, int64_t count, void *stream);
 
 /* Exhaustive autotuner: time every candidate configuration of each instance
- * (as a batch of that instance repeated up to min_ops operations, results in
- * scratch so caller data is untouched) and store the fastest in inst[i].cfg.
+ * (as batches of that instance repeated R and 2R times, R x ops about
+ * min_ops, results in scratch so caller data is untouched) and store the one
+ * with the lowest marginal cost in inst[i].cfg: the time the R extra copies
+ * add per operation, which is what an instance costs inside a large batch.
  * costs (optional): count x max_cand seconds per operation, -1 where unused. */
 int bale_tune(bale_instance *inst, int64_t count, int64_t min_ops, float *costs, int max_cand,
               void *stream);
