@@ -3,8 +3,10 @@
 // instance baseline.
 //
 // A batch is planned on the host into task lists, one per phase:
-//   A  (persistent, 512 threads, 32 KiB LDS)  gather / direct / direct+hot /
-//      LDS-tile tasks of every instance, and the count pass of bucketed ones
+//   A  (persistent, 512 threads, 32 KiB LDS)  gather (straight from the table,
+//      or from an LDS copy of a table of at most 4096 entries) / direct /
+//      direct+hot / LDS-tile tasks of every instance, and the count pass of
+//      bucketed ones
 //   B  (one workgroup per bucketed instance)   chunk offsets + bucket scan
 //   C  (persistent, 1024 threads)               bucket scatter, one task per chunk
 //   D  (persistent, 1024 threads, 128 KiB LDS)  bucket apply, one task per segment slot
@@ -33,6 +35,7 @@ enum : uint32_t {
   OP_HLDS = 4,
   OP_HLDS_RUNS = 5,
   OP_BCOUNT = 6,
+  OP_GLDS = 7,
 };
 constexpr uint32_t kFlagHot = 1, kFlagRuns = 2;
 constexpr int kBlockA = 512;
@@ -84,6 +87,40 @@ __device__ inline uint32_t wave_merge(uint64_t key, bool ok) {
     mine = (uint32_t)(end - lane);
   }
   return mine;
+}
+
+// Gather from an LDS copy of the table (at most kLdsAWords / 2 entries): the
+// table is read once per task, then every lookup is an LDS read.
+__device__ inline void task_glds(uint32_t* sh, const BInst& I, int64_t b, int64_t e, ErrWords* err) {
+  constexpr int U = 8;
+  const int T = blockDim.x, t = threadIdx.x;
+  int64_t* st = (int64_t*)sh;
+  const int64_t* __restrict__ table = (const int64_t*)I.table;
+  for (int64_t k = t; k < I.table_len; k += T) st[k] = table[k];
+  __syncthreads();
+  const int64_t* __restrict__ idx = I.idx;
+  int64_t* __restrict__ tgt = I.tgt;
+  unsigned br = 0, bp = 0;
+  for (int64_t base = b; base < e; base += (int64_t)T * U) {
+    int64_t raw[U], v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      int64_t i = base + (int64_t)k * T + t;
+      raw[k] = i < e ? __builtin_nontemporal_load(idx + i) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      int64_t i = base + (int64_t)k * T + t;
+      uint64_t j;
+      v[k] = (i < e && decode_index<false>(raw[k], (uint64_t)I.table_len, j, br, bp)) ? st[j] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      int64_t i = base + (int64_t)k * T + t;
+      if (i < e) __builtin_nontemporal_store(v[k], tgt + i);
+    }
+  }
+  flush_errors(err, br, bp);
 }
 
 __device__ inline void task_gather(const BInst& I, int64_t b, int64_t e, ErrWords* err) {
@@ -230,6 +267,7 @@ __global__ __launch_bounds__(kBlockA) void k_batch_a(const BInst* __restrict__ i
     const uint32_t op = T.op & 0xff, arg = T.op >> 8;
     switch (op) {
       case OP_GATHER: task_gather(I, T.begin, T.end, err); break;
+      case OP_GLDS: task_glds(sh, I, T.begin, T.end, err); break;
       case OP_HDIRECT: task_hdirect<0>(I, T.begin, T.end, err); break;
       case OP_HDIRECT_RUNS: task_hdirect<1>(I, T.begin, T.end, err); break;
       case OP_HHOT: task_hhot(sh, I, T.begin, T.end, err); break;
@@ -477,7 +515,8 @@ int plan_add(Plan& P, const bale_instance& in, const Device* d) {
       P.a.push_back(BTask{id, (uint32_t)b, (uint32_t)std::min(in.n, b + task), op});
   };
   if (in.kind == 1) {
-    add_range_tasks(OP_GATHER);
+    const bool in_lds = c.strategy == BALE_STRAT_LDS && in.table_len <= kLdsAWords / 2;
+    add_range_tasks(in_lds ? OP_GLDS : OP_GATHER);
   } else if (c.strategy == BALE_STRAT_LDS &&
              ceil_div(in.table_len, (int64_t)1 << kLdsTileBits) <= 64) {
     int tb = std::max(8, std::min(kLdsTileBits, ceil_log2(in.table_len)));
@@ -767,6 +806,8 @@ int bale_tune_candidates(int kind, int64_t table_len, bale_config* out, int max)
   };
   if (kind == 1) {
     for (int t : {14, 16, 18}) add(BALE_STRAT_DIRECT, t, 0, 0);
+    if (table_len <= kLdsAWords / 2)
+      for (int t : {14, 16, 18}) add(BALE_STRAT_LDS, t, 0, 0);
   } else {
     for (int t : {14, 16, 18}) add(BALE_STRAT_DIRECT, t, 0, 0);
     add(BALE_STRAT_DIRECT, 16, 1, 0);

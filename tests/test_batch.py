@@ -24,6 +24,9 @@ def test_candidates_cover_the_strategies():
     assert strats == {B.DIRECT, B.LDS, B.BUCKET}
     assert {c.strategy for c in M.candidates(0, 1 << 20)} == {B.DIRECT, B.BUCKET}
     assert {c.strategy for c in M.candidates(1, 1 << 20)} == {B.DIRECT}
+    # gathers from tables of at most 4096 entries can stage the table in LDS
+    assert {c.strategy for c in M.candidates(1, 4096)} == {B.DIRECT, B.LDS}
+    assert {c.strategy for c in M.candidates(1, 4097)} == {B.DIRECT}
     assert any(c.hot for c in c_small) and any(c.merge for c in c_small)
 
 
@@ -60,6 +63,23 @@ def test_batch_every_candidate_matches_oracle(dev, orc):
     b.launch_direct()
     B.sync_errors()
     _check_mix(mix, orc, 3)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batch_gathers_from_lds(dev, orc):
+    """Gathers configured LDS read an LDS copy of tables of at most 4096
+    entries (1, 7, 4096); a larger table (4097) runs the plain gather."""
+    params = [M.MixParams(1, d, tl, n, 1.1, 9000 + k)
+              for k, (tl, n, d) in enumerate([(1, 5000, 0), (7, 70000, 1), (4096, 1 << 18, 0),
+                                               (4096, 333, 2), (4097, 1 << 17, 0)])]
+    mix = M.Mix(params, dev)
+    for i in range(len(params)):
+        mix.inst[i].cfg = B.config(B.LDS, task_log2=14 + i % 3)
+    b = M.Batch(mix.inst, len(params))
+    b.launch()
+    B.sync_errors()
+    _check_mix(mix, orc, 1)
     b.close()
 
 
