@@ -42,6 +42,7 @@ constexpr int kBlockA = 512;
 constexpr int kLdsAWords = 8192;  // 32 KiB: LDS tiles of up to 2^13 bins
 constexpr int kLdsTileBits = 13;
 constexpr int kBucketBits = 15;   // apply tiles of 2^15 bins (128 KiB of LDS)
+constexpr int kXcds = 8;          // MI355X: 8 XCDs, each with its own L2
 
 struct BInst {
   void* table;          // counts (histogram) or table (gather)
@@ -247,17 +248,31 @@ __device__ inline void task_hlds(uint32_t* sh, const BInst& I, int64_t b, int64_
   if (tile == 0) flush_errors(err, br, bp);
 }
 
-// Phase A.
+// Phase A: a persistent loop over tasks.  Queue q holds tasks[qoff[q],
+// qoff[q+1]); a workgroup takes from queue blockIdx % nq until it is empty,
+// then from the next ones in turn, so the loop ends once all nq queues are
+// drained (each head only grows).
 __global__ __launch_bounds__(kBlockA) void k_batch_a(const BInst* __restrict__ insts,
                                                      const BTask* __restrict__ tasks,
-                                                     uint32_t ntasks, uint32_t* __restrict__ head,
-                                                     ErrWords* err) {
+                                                     const uint32_t* __restrict__ qoff, uint32_t nq,
+                                                     uint32_t* __restrict__ head, ErrWords* err) {
   extern __shared__ uint32_t sh[];
   __shared__ BTask s_t;
+  const uint32_t q0 = blockIdx.x % nq;
+  uint32_t tried = 0;  // thread 0: queues found empty so far
   while (true) {
     if (threadIdx.x == 0) {
-      uint32_t k = atomicAdd(head, 1u);
-      s_t = k < ntasks ? tasks[k] : BTask{~0u, 0, 0, 0};
+      BTask t{~0u, 0, 0, 0};
+      while (tried < nq) {
+        const uint32_t q = (q0 + tried) % nq;
+        const uint32_t k = atomicAdd(head + q, 1u);
+        if (k < qoff[q + 1] - qoff[q]) {
+          t = tasks[qoff[q] + k];
+          break;
+        }
+        tried++;
+      }
+      s_t = t;
     }
     __syncthreads();
     const BTask T = s_t;
@@ -467,13 +482,14 @@ struct Plan {
   std::vector<BInst> inst;
   std::vector<Offs> off;
   std::vector<BTask> a, c, d;
+  std::vector<uint32_t> qoff;  // phase-A queue q (one per XCD) holds a[qoff[q], qoff[q+1])
   std::vector<uint32_t> hot_list, bucket_list;
   size_t arena = 0;
   size_t lds_c = 0;
 };
 
 struct Buffers {
-  DevBuf arena, inst, a, c, d, hot_list, bucket_list, heads;
+  DevBuf arena, inst, a, qoff, c, d, hot_list, bucket_list, heads;
 };
 
 // Default per-instance rule when cfg.strategy is AUTO.
@@ -593,12 +609,38 @@ int plan_upload(Plan& P, Buffers& Bf, hipStream_t s) {
       I.staged = (uint16_t*)(base + o.staged);
     }
   }
-  // largest tasks first: bucket counts (they gate phases B-D), then by length
-  std::stable_sort(P.a.begin(), P.a.end(), [](const BTask& x, const BTask& y) {
-    bool cx = (x.op & 0xff) == OP_BCOUNT, cy = (y.op & 0xff) == OP_BCOUNT;
-    if (cx != cy) return cx;
-    return (x.end - x.begin) > (y.end - y.begin);
-  });
+  // Phase A runs from one queue per XCD (workgroup b is dispatched to XCD
+  // b % kXcds and drains that queue first, then the others): whole instances
+  // go to the least-loaded queue, heaviest first, and each queue lists the
+  // bucket counts first (they gate phases B-D), then its instances' tasks
+  // together, heaviest instance first.  An XCD's L2 then holds the tables of
+  // a few instances at a time, not of every instance in flight.
+  {
+    const size_t ni = P.inst.size();
+    std::vector<int64_t> w(ni, 0);
+    for (const BTask& t : P.a) w[t.inst] += (int64_t)(t.end - t.begin);
+    std::vector<uint32_t> ord(ni), q(ni, 0);
+    for (size_t i = 0; i < ni; i++) ord[i] = (uint32_t)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return w[x] > w[y]; });
+    int64_t load[kXcds] = {};
+    for (uint32_t i : ord) {
+      int best = 0;
+      for (int k = 1; k < kXcds; k++)
+        if (load[k] < load[best]) best = k;
+      q[i] = (uint32_t)best;
+      load[best] += w[i];
+    }
+    std::stable_sort(P.a.begin(), P.a.end(), [&](const BTask& x, const BTask& y) {
+      if (q[x.inst] != q[y.inst]) return q[x.inst] < q[y.inst];
+      bool cx = (x.op & 0xff) == OP_BCOUNT, cy = (y.op & 0xff) == OP_BCOUNT;
+      if (cx != cy) return cx;
+      if (w[x.inst] != w[y.inst]) return w[x.inst] > w[y.inst];
+      return x.inst < y.inst;
+    });
+    P.qoff.assign(kXcds + 1, 0);
+    for (const BTask& t : P.a) P.qoff[q[t.inst] + 1]++;
+    for (int k = 0; k < kXcds; k++) P.qoff[k + 1] += P.qoff[k];
+  }
   auto up = [&](DevBuf& b, const void* src, size_t bytes) -> int {
     int r = b.ensure(std::max<size_t>(bytes, 16));
     if (r) return r;
@@ -607,6 +649,7 @@ int plan_upload(Plan& P, Buffers& Bf, hipStream_t s) {
   };
   if ((rc = up(Bf.inst, P.inst.data(), P.inst.size() * sizeof(BInst)))) return rc;
   if ((rc = up(Bf.a, P.a.data(), P.a.size() * sizeof(BTask)))) return rc;
+  if ((rc = up(Bf.qoff, P.qoff.data(), P.qoff.size() * 4))) return rc;
   if ((rc = up(Bf.c, P.c.data(), P.c.size() * sizeof(BTask)))) return rc;
   if ((rc = up(Bf.d, P.d.data(), P.d.size() * sizeof(BTask)))) return rc;
   if ((rc = up(Bf.hot_list, P.hot_list.data(), P.hot_list.size() * 4))) return rc;
@@ -630,14 +673,14 @@ void set_batch_lds(const Plan& P) {
 void plan_enqueue(const Plan& P, Buffers& Bf, Device* d, hipStream_t s) {
   const BInst* insts = (const BInst*)Bf.inst.p;
   uint32_t* heads = (uint32_t*)Bf.heads.p;
-  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, heads, 4);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, heads, 4 + kXcds);
   if (!P.hot_list.empty())
     hipLaunchKernelGGL(k_batch_sketch, dim3((unsigned)P.hot_list.size()), dim3(1024), 0, s, insts,
                        (const uint32_t*)Bf.hot_list.p);
   if (!P.a.empty()) {
     int64_t g = std::min<int64_t>((int64_t)d->cus * 4, (int64_t)P.a.size());
     hipLaunchKernelGGL(k_batch_a, dim3((unsigned)g), dim3(kBlockA), (size_t)kLdsAWords * 4, s, insts,
-                       (const BTask*)Bf.a.p, (uint32_t)P.a.size(), heads + 0, d->err);
+                       (const BTask*)Bf.a.p, (const uint32_t*)Bf.qoff.p, (uint32_t)kXcds, heads + 4, d->err);
   }
   if (!P.bucket_list.empty()) {
     hipLaunchKernelGGL(k_batch_b, dim3((unsigned)P.bucket_list.size()), dim3(256), 0, s, insts,
