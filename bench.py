@@ -249,9 +249,36 @@ def cpu_baseline(idx_t, table_t, max_threads=16):
     for _ in range(3):
         secs.append(oracle.ig_agi_mt(idx, table, tgt, threads))
     t = sum(secs) / len(secs)
+    # one thread (P = 1) on a smaller sample of the same requests
+    s1 = min(n, 1 << 23)
+    oracle.ig_agi_mt(idx[:s1], table, tgt[:s1], 1)
+    t1 = sum(oracle.ig_agi_mt(idx[:s1], table, tgt[:s1], 1) for _ in range(3)) / 3
     return {"value": sample / t / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
             "sample": f"first 2^{sample.bit_length() - 1} of the 2^28 requests into the full 2^28-entry "
-                      f"table (2 GiB), oracle or_ig_agi_mt with {threads} threads, mean of 3 after 1 warm-up"}
+                      f"table (2 GiB), oracle or_ig_agi_mt with {threads} threads, mean of 3 after 1 warm-up",
+            "one_thread": {"value": s1 / t1 / 1e9, "unit": "GUPS", "cores": 1,
+                           "sample": f"first 2^{s1.bit_length() - 1} requests, same table"}}
+
+
+def cpu_histo(max_threads=16):
+    """The oracle's AGI histogram (relaxed atomic adds on shared counters,
+    apps/histo_src/histo_agi.upc:61-66) on the host cores and on one core:
+    uniform updates into 2^26 int64 counters, the bench's 2^28-update case
+    sampled to 2^26 (2^24 for one thread)."""
+    import numpy as np
+    import oracle
+    threads = max(1, min(max_threads, len(os.sched_getaffinity(0))))
+    nbins = 1 << 26
+    idx = oracle.gen_index(1 << 26, nbins, 0, 1.0, 0, 3)
+    counts = np.zeros(nbins, np.int64)
+    res = {}
+    for cores, n in ((threads, idx.size), (1, 1 << 24)):
+        oracle.histo_agi_mt(idx[:n], counts, cores)
+        t = sum(oracle.histo_agi_mt(idx[:n], counts, cores) for _ in range(3)) / 3
+        res[f"{cores}_threads"] = {"value": n / t / 1e9, "unit": "GUPS", "cores": cores, "kind": "port",
+                                   "sample": f"2^{n.bit_length() - 1} uniform updates (seed 3) into 2^26 "
+                                             f"int64 counters, mean of 3 after 1 warm-up"}
+    return res
 
 
 def load_traffic(name: str):
@@ -498,6 +525,9 @@ def main() -> int:
             histo[name] = r
             log(f"{name}: {r['GUPS']:.2f} GUPS cfg={r['cfg']}")
             torch.cuda.empty_cache()
+        if not args.no_cpu:
+            histo["cpu_baseline"] = cpu_histo()
+            log(f"histo cpu: {histo['cpu_baseline']}")
         extra["histo"] = histo
         log("conveyor 2^26 x 8 B -> 64 PEs ...")
         extra["convey"] = bench_convey(B, torch, dev, stream)
