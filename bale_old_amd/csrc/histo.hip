@@ -1030,11 +1030,10 @@ static int direct_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   int64_t per = (int64_t)block * U;
   int64_t grid = ceil_div(n, per);
   if (c.hot && nbins < (int64_t)kHotEmpty) {
-    void* ws = workspace(d, kHotSlots * 4, s);
+    void* ws = workspace(d, (kHotSlots + kSketchScratch) * 4, s);
     if (!ws) return (int)hipErrorOutOfMemory;
     uint32_t* hot = (uint32_t*)ws;
-    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins, hot,
-                       (uint32_t*)nullptr);
+    launch_hot_sketch<PACKED>(idx, n, (uint64_t)nbins, hot, nullptr, hot + kHotSlots, s);
     int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : (block == 1024 ? 2 : 4);
     grid = std::min<int64_t>(grid, (int64_t)d->cus * gpc);
     dim3 g((unsigned)grid), b(block);
@@ -1158,8 +1157,7 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   // worth handling; every pass reads its verdict from the flag word
   uint32_t* flags = c.hot == 2 ? L.misc + (kMiscWords - 1) : nullptr;
   if (HOT)
-    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins, L.hot,
-                       flags);
+    launch_hot_sketch<PACKED>(idx, n, (uint64_t)nbins, L.hot, flags, L.sketch, s);
   size_t lds_count = (size_t)(B + (HOT ? kHotSlots : 0)) * 4;
   hipLaunchKernelGGL((k_part_count<PACKED, HOT, RUNS>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
                      n, (uint64_t)nbins, tb, (int)B, chunk, L.chunk, L.hot, d->err,
@@ -1239,8 +1237,7 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   uint16_t* staged = (uint16_t*)((char*)ws + head + 2 * tag_b + used_b);
   uint32_t* flags = adaptive ? L.misc + (kMiscWords - 1) : nullptr;
   if (HOT)
-    hipLaunchKernelGGL(k_hot_sketch<PACKED>, dim3(1), dim3(1024), 0, s, idx, n, (uint64_t)nbins,
-                       L.hot, flags);
+    launch_hot_sketch<PACKED>(idx, n, (uint64_t)nbins, L.hot, flags, L.sketch, s);
   size_t lds = ring_scatter_lds((int)B, R, HOT);
   if (adaptive) lds = std::max(lds, ring_scatter_lds((int)B, R_cold, false));
   allow_lds(k_histo_scatter_ring<PACKED, HOT, RUNS, CT>, lds);

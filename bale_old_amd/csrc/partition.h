@@ -59,11 +59,15 @@ struct PartLayout {
   uint32_t* segstart;  // [B+1] first apply segment of each bucket
   uint32_t* misc;      // [kMiscWords] queue totals and heads
   uint32_t* hot;       // [kHotSlots] hot-bin hash table (keys)
+  uint32_t* sketch;    // [kSketchScratch] sampled keys and pair codes
   void* staged;        // [n]   entries grouped by bucket
 };
 
+// Scratch of the two-kernel sketch: the sampled keys, then one pair code per sample.
+constexpr int kSketchScratch = 2 * kSketch;
+
 inline size_t part_words(int64_t C, int64_t B) {
-  return (size_t)(C * B + (B + 1) + B + (B + 1) + kMiscWords + kHotSlots);
+  return (size_t)(C * B + (B + 1) + B + (B + 1) + kMiscWords + kHotSlots + kSketchScratch);
 }
 inline size_t part_bytes(int64_t C, int64_t B, int64_t n, size_t esize) {
   size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
@@ -78,6 +82,7 @@ inline PartLayout part_carve(void* ws, int64_t C, int64_t B) {
   L.segstart = L.total + B;
   L.misc = L.segstart + (B + 1);
   L.hot = L.misc + kMiscWords;
+  L.sketch = L.hot + kHotSlots;
   size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
   L.staged = (char*)ws + head;
   return L;
@@ -99,42 +104,55 @@ __device__ inline uint32_t sketch_hash(uint32_t v) {
 }
 static_assert(kSketchSlots == (1 << 13), "sketch_hash assumes 2^13 slots");
 
+// Sample k of kSketch sits at position floor(k * n / kSketch).
+__device__ inline int64_t sketch_pos(int k, int64_t n) {
+  return (int64_t)(((unsigned __int128)k * (uint64_t)n) / kSketch);
+}
+
+// Key of one sample (kHotEmpty when it is out of range or cannot be a hot
+// bin) and its pair code: bit 0 = it has a right neighbour, bit 1 = the
+// neighbour names the same bin.
 template <bool PACKED>
-__device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t n, uint64_t limit,
-                                       uint32_t* __restrict__ hot_out,
-                                       uint32_t* __restrict__ flags = nullptr) {
+__device__ inline uint32_t sketch_key(int64_t pos, int64_t n, uint64_t limit, int64_t r0, int64_t r1,
+                                      bool pairs, uint32_t& code) {
+  uint64_t j = 0, j2 = 0;
+  unsigned br = 0, bp = 0;
+  code = 0;
+  const bool ok = pos < n && decode_index<PACKED>(r0, limit, j, br, bp);
+  if (pairs && ok && pos + 1 < n)
+    code = 1u | ((decode_index<PACKED>(r1, limit, j2, br, bp) && j2 == j) ? 2u : 0u);
+  return ok && j < kHotEmpty ? (uint32_t)j : kHotEmpty;
+}
+
+// Sum of v over the 64 lanes of the wave (every lane gets it).
+__device__ inline uint32_t wave_add(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// From the kSketch sampled keys (kPer per thread of a 1024-thread workgroup,
+// sample tid + q * 1024 in key[q]) and the workgroup's pair counts: the hash
+// table of bins that occur at least twice and, with `flags`, the verdict.
+__device__ inline void sketch_tables(const uint32_t* key, uint32_t eq, uint32_t pairs, uint64_t limit,
+                                     uint32_t* __restrict__ hot_out, uint32_t* __restrict__ flags) {
   __shared__ uint32_t keys[kSketchSlots];
   __shared__ uint32_t cnt2[kSketchSlots / 2];  // two 16-bit counts per word
   __shared__ uint32_t table[kHotSlots];
   __shared__ uint32_t nhot, neq, npairs, ndup;
+  constexpr int kPer = kSketch / 1024;
   const int tid = threadIdx.x, nthr = blockDim.x;
   for (int k = tid; k < kSketchSlots; k += nthr) keys[k] = kHotEmpty;
   for (int k = tid; k < kSketchSlots / 2; k += nthr) cnt2[k] = 0;
   for (int k = tid; k < kHotSlots; k += nthr) table[k] = kHotEmpty;
   if (tid == 0) nhot = neq = npairs = ndup = 0;
   __syncthreads();
-  uint32_t eq = 0, pairs = 0;
-  // every sample (and its right neighbour) is loaded before any is used
-  constexpr int kPer = kSketch / 1024;  // launched with 1024 threads
-  int64_t r0s[kPer], r1s[kPer];
 #pragma unroll
   for (int q = 0; q < kPer; q++) {
-    const int k = tid + q * 1024;
-    const int64_t pos = (int64_t)(((unsigned __int128)k * (uint64_t)n) / kSketch);
-    r0s[q] = pos < n ? idx[pos] : 0;
-    r1s[q] = flags && pos + 1 < n ? idx[pos + 1] : 0;
-  }
-#pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const int k = tid + q * 1024;
-    const int64_t pos = (int64_t)(((unsigned __int128)k * (uint64_t)n) / kSketch);
-    const int64_t r0 = r0s[q], r1 = r1s[q];
-    uint64_t j = 0, j2 = 0;
-    unsigned br = 0, bp = 0;
-    bool ok = pos < n && decode_index<PACKED>(r0, limit, j, br, bp);
-    if (ok && j < kHotEmpty) {
+    const uint32_t v = key[q];
+    if (v != kHotEmpty) {
       // at most kSketch keys in 2 * kSketch slots: the probe always ends
-      uint32_t v = (uint32_t)j, h = sketch_hash(v);
+      uint32_t h = sketch_hash(v);
       for (;;) {
         uint32_t prev = atomicCAS(&keys[h], kHotEmpty, v);
         if (prev == kHotEmpty || prev == v) break;
@@ -142,14 +160,15 @@ __device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t 
       }
       atomicAdd(&cnt2[h >> 1], 1u << (16 * (h & 1)));  // counts stay below 2^16
     }
-    if (flags && ok && pos + 1 < n) {
-      pairs++;
-      if (decode_index<PACKED>(r1, limit, j2, br, bp) && j2 == j) eq++;
-    }
   }
   if (flags) {
-    atomicAdd(&neq, eq);
-    atomicAdd(&npairs, pairs);
+    // one LDS atomic per wave: same-address atomics from every lane serialise
+    eq = wave_add(eq);
+    pairs = wave_add(pairs);
+    if ((tid & 63) == 0) {
+      atomicAdd(&neq, eq);
+      atomicAdd(&npairs, pairs);
+    }
   }
   __syncthreads();
   // a bin is hot when it occurs at least twice in the sample
@@ -165,7 +184,10 @@ __device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t 
       }
     }
   }
-  if (flags) atomicAdd(&ndup, dup);
+  if (flags) {
+    dup = wave_add(dup);
+    if ((tid & 63) == 0) atomicAdd(&ndup, dup);
+  }
   __syncthreads();
   for (int k = tid; k < kHotSlots; k += nthr) hot_out[k] = table[k];
   if (flags && tid == 0) {
@@ -179,11 +201,76 @@ __device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t 
   __syncthreads();
 }
 
+// The sketch in one workgroup of 1024 (the batch runs one per instance, side
+// by side): every sample (and its right neighbour) is loaded before any is used.
 template <bool PACKED>
-__global__ __launch_bounds__(1024) void k_hot_sketch(const int64_t* __restrict__ idx, int64_t n,
-                                                     uint64_t limit, uint32_t* __restrict__ hot_out,
-                                                     uint32_t* __restrict__ flags) {
-  hot_sketch_body<PACKED>(idx, n, limit, hot_out, flags);
+__device__ inline void hot_sketch_body(const int64_t* __restrict__ idx, int64_t n, uint64_t limit,
+                                       uint32_t* __restrict__ hot_out,
+                                       uint32_t* __restrict__ flags = nullptr) {
+  constexpr int kPer = kSketch / 1024;
+  const int tid = threadIdx.x;
+  int64_t r0s[kPer], r1s[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int64_t pos = sketch_pos(tid + q * 1024, n);
+    r0s[q] = pos < n ? idx[pos] : 0;
+    r1s[q] = flags && pos + 1 < n ? idx[pos + 1] : 0;
+  }
+  uint32_t key[kPer], eq = 0, pairs = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    uint32_t code;
+    key[q] = sketch_key<PACKED>(sketch_pos(tid + q * 1024, n), n, limit, r0s[q], r1s[q], flags != nullptr, code);
+    pairs += code & 1u;
+    eq += code >> 1;
+  }
+  sketch_tables(key, eq, pairs, limit, hot_out, flags);
+}
+
+// A single workgroup's scattered loads are bound by the misses one CU keeps
+// in flight (~20 us for the 4096 samples and their neighbours).  Standalone
+// calls spread the loads instead: k_sketch_sample (kSketchGroups workgroups,
+// one sample per thread) writes each sample's key and pair code to scratch,
+// and k_sketch_build (one workgroup) reads them back as streams.
+constexpr int kSketchGroups = kSketch / 256;
+
+template <bool PACKED>
+__global__ __launch_bounds__(256) void k_sketch_sample(const int64_t* __restrict__ idx, int64_t n,
+                                                       uint64_t limit, int pairs,
+                                                       uint32_t* __restrict__ scratch) {
+  const int k = blockIdx.x * 256 + threadIdx.x;  // grid kSketchGroups x 256 covers every sample
+  const int64_t pos = sketch_pos(k, n);
+  const int64_t r0 = pos < n ? idx[pos] : 0;
+  const int64_t r1 = pairs && pos + 1 < n ? idx[pos + 1] : 0;
+  uint32_t code;
+  scratch[k] = sketch_key<PACKED>(pos, n, limit, r0, r1, pairs != 0, code);
+  scratch[kSketch + k] = code;
+}
+
+static __global__ __launch_bounds__(1024) void k_sketch_build(const uint32_t* __restrict__ scratch, uint64_t limit,
+                                                       uint32_t* __restrict__ hot_out,
+                                                       uint32_t* __restrict__ flags) {
+  constexpr int kPer = kSketch / 1024;
+  uint32_t key[kPer], eq = 0, pairs = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int k = threadIdx.x + q * 1024;
+    key[q] = scratch[k];
+    const uint32_t code = scratch[kSketch + k];
+    pairs += code & 1u;
+    eq += code >> 1;
+  }
+  sketch_tables(key, eq, pairs, limit, hot_out, flags);
+}
+
+// Enqueue the two-kernel sketch; scratch holds kSketchScratch words.
+template <bool PACKED>
+inline void launch_hot_sketch(const int64_t* idx, int64_t n, uint64_t limit, uint32_t* hot_out,
+                              uint32_t* flags, uint32_t* scratch, hipStream_t s) {
+  hipLaunchKernelGGL(k_sketch_sample<PACKED>, dim3(kSketchGroups), dim3(256), 0, s, idx, n, limit,
+                     flags ? 1 : 0, scratch);
+  hipLaunchKernelGGL(k_sketch_build, dim3(1), dim3(1024), 0, s, (const uint32_t*)scratch, limit, hot_out,
+                     flags);
 }
 
 // Runs of equal keys in adjacent lanes of the wave.  Returns the lane that heads
