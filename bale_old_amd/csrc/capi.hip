@@ -118,9 +118,13 @@ static void plan_histo(const Device* d, int64_t n, int64_t nbins, int dtype, bal
     c->strategy = BALE_STRAT_BUCKET;
     c->block = 1024;
     c->hot = 2;
-    // bucket width: narrower buckets flush cheaper on small tables, wider ones
-    // (2^16, packed 16-bit apply tile) halve the scatter's buckets on big ones
-    c->tile_log2 = nbins <= ((int64_t)1 << 22) ? 14 : nbins > ((int64_t)1 << 25) ? 16 : 0;
+    // bucket width: about 256 buckets (one per CU in the apply pass), within
+    // 2^10 bins and the packed 16-bit tile's 2^16; measured on MI355X against
+    // the earlier fixed 2^14 (up to 2^22 bins) and 2^15/2^16 widths: 1.0-1.7x
+    // on uniform inputs, 0.85-1.2x on Zipf(1.1) ones
+    int lb = 0;
+    while (((int64_t)1 << lb) < nbins) lb++;
+    c->tile_log2 = std::max(10, std::min(16, lb - 8));
   } else {
     c->strategy = BALE_STRAT_DIRECT;
     c->block = 256;

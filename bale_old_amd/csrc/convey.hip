@@ -76,8 +76,8 @@ static int convey_deliver(bale_convey* c, hipStream_t s) {
                      L.chunk, (const uint32_t*)nullptr, d->err, (const uint32_t*)nullptr);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)npe), dim3(256), 0, s, L.chunk, (int)C, npe,
                      L.total);
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, npe, 1u << 30, L.start,
-                     L.segstart, L.misc);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, npe, 1u << 30, 0u,
+                     (const uint32_t*)nullptr, L.start, L.segstart, L.misc);
   const dim3 g((unsigned)C);
   switch (c->item_bytes / 4) {
     case 1: launch_deliver<1>(g, s, c->queue, c->qpes, n, npe, nbits, chunk, L.chunk, L.start, c->out); break;

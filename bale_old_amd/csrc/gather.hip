@@ -279,8 +279,8 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
   int gr = c.groups_per_cu > 0 ? c.groups_per_cu : 2;
   const int64_t per_xcd = std::max<int64_t>(1, (int64_t)d->cus * gr / kXcds);
   int64_t seg = std::max<int64_t>(2048, ceil_div(ceil_div(n, B), per_xcd));
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
-                     L.start, L.segstart, L.misc);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg, 0u,
+                     (const uint32_t*)nullptr, L.start, L.segstart, L.misc);
   uint32_t* xseg = L.segstart;  // reused: per-XCD segment starts
   hipLaunchKernelGGL(k_gather_xseg, dim3(1), dim3(64), 0, s, L.total, (int)B, (uint32_t)seg, xseg,
                      L.misc);

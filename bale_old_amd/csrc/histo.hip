@@ -392,7 +392,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
                                                       const uint32_t* __restrict__ start,
                                                       const uint32_t* __restrict__ segstart,
                                                       uint32_t* __restrict__ misc, int B,
-                                                      uint32_t seg_len, int shift, uint64_t nbins,
+                                                      int shift, uint64_t nbins,
                                                       CT* __restrict__ counts,
                                                       const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
                    spill_base + (1u << shift) / 32u + kSpillSlots};
   const bool merge = RUNS && (!flags || (*flags & kFlagRunsSeen));
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const uint32_t total = misc[0];
+  const uint32_t total = misc[0], seg_len = misc[kSegWord];
   while (true) {
     if (tid == 0) {
       uint32_t w = atomicAdd(&misc[1], 1u);
@@ -913,7 +913,7 @@ template <typename CT, bool HOT, bool RUNS, bool PK>
 __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     const uint16_t* __restrict__ staged, const uint32_t* __restrict__ tags,
     const uint32_t* __restrict__ plist, const uint32_t* __restrict__ start,
-    const uint32_t* __restrict__ segstart, uint32_t* __restrict__ misc, int B, uint32_t seg_pages,
+    const uint32_t* __restrict__ segstart, uint32_t* __restrict__ misc, int B,
     int shift, uint64_t nbins, int S, int S_cold, CT* __restrict__ counts,
     const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
@@ -928,7 +928,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
   const uint32_t vpp = Se / 8u;  // 16-byte vectors per page (a power of two)
   const int vsh = __builtin_ctz(vpp);
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const uint32_t total = misc[0];
+  const uint32_t total = misc[0], seg_pages = misc[kSegWord];
   while (true) {
     if (tid == 0) {
       uint32_t w = atomicAdd(&misc[1], 1u);
@@ -1128,6 +1128,17 @@ static int64_t apply_segment(int64_t n, int tb, const bale_config& c, const Devi
   return std::max<int64_t>({4096, ceil_div(n, (int64_t)d->cus * 2), (int64_t)4 << tb});
 }
 
+// Apply segment for inputs the sketch finds without hot bins (taken on the
+// device): twice the average bucket, rounded up to a power of two, so nearly
+// every bucket is one segment.  0 (no alternative) when the caller fixed the
+// segment length.
+static int64_t uniform_segment(int64_t n, int64_t B, const bale_config& c) {
+  if (c.task_log2 > 0) return 0;
+  int64_t v = 4096;
+  while (v < 2 * ceil_div(n, B) && v < ((int64_t)1 << 30)) v <<= 1;
+  return v;
+}
+
 template <bool PACKED, bool HOT, bool RUNS, typename CT>
 static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins,
                         const bale_config& c, Device* d, hipStream_t s) {
@@ -1165,8 +1176,9 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);
   int64_t seg = apply_segment(n, tb, c, d);
+  const int64_t seg_alt = uniform_segment(n, B, c);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
-                     L.start, L.segstart, L.misc);
+                     (uint32_t)seg_alt, (const uint32_t*)flags, L.start, L.segstart, L.misc);
   size_t lds = part_scatter_lds((int)B, S, sizeof(uint16_t), HOT);
   // adaptive mode: when the sample finds no hot bins, the scatter pass puts
   // its buffers over the hot table, as deep as the no-hot layout allows
@@ -1185,12 +1197,12 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
     allow_lds(k_histo_apply<CT, RUNS, true>, lds_apply);
     hipLaunchKernelGGL((k_histo_apply<CT, RUNS, true>), dim3((unsigned)groups), dim3(1024),
                        lds_apply, s, (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B,
-                       (uint32_t)seg, tb, (uint64_t)nbins, counts, (const uint32_t*)flags);
+                       tb, (uint64_t)nbins, counts, (const uint32_t*)flags);
   } else {
     allow_lds(k_histo_apply<CT, RUNS, false>, lds_apply);
     hipLaunchKernelGGL((k_histo_apply<CT, RUNS, false>), dim3((unsigned)groups), dim3(1024),
                        lds_apply, s, (const uint16_t*)L.staged, L.start, L.segstart, L.misc, (int)B,
-                       (uint32_t)seg, tb, (uint64_t)nbins, counts, (const uint32_t*)flags);
+                       tb, (uint64_t)nbins, counts, (const uint32_t*)flags);
   }
   return (int)hipGetLastError();
 }
@@ -1249,8 +1261,11 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   // apply segments in pages of the hot-on depth
   const int64_t seg = apply_segment(n, tb, c, d);
   const uint32_t seg_pages = (uint32_t)ceil_div(seg, P);
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, seg_pages, L.start,
-                     L.segstart, L.misc);
+  // the uniform alternative is only taken without hot bins: cold pages
+  const uint32_t seg_alt_pages =
+      adaptive ? (uint32_t)ceil_div(uniform_segment(n, B, c), (int64_t)P_cold) : 0u;
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, seg_pages, seg_alt_pages,
+                     (const uint32_t*)flags, L.start, L.segstart, L.misc);
   allow_lds(k_page_list<HOT>, page_list_lds((int)B));
   hipLaunchKernelGGL(k_page_list<HOT>, dim3((unsigned)C), dim3(1024), page_list_lds((int)B), s,
                      (const uint32_t*)tags, (const uint32_t*)L.chunk, (const uint32_t*)L.start,
@@ -1262,14 +1277,14 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
     hipLaunchKernelGGL((k_histo_apply_paged<CT, HOT, RUNS, true>), dim3((unsigned)groups),
                        dim3(1024), lds_apply, s, (const uint16_t*)staged, (const uint32_t*)tags,
                        (const uint32_t*)plist, (const uint32_t*)L.start, (const uint32_t*)L.segstart,
-                       L.misc, (int)B, seg_pages, tb, (uint64_t)nbins, P, P_cold, counts,
+                       L.misc, (int)B, tb, (uint64_t)nbins, P, P_cold, counts,
                        (const uint32_t*)flags);
   } else {
     allow_lds(k_histo_apply_paged<CT, HOT, RUNS, false>, lds_apply);
     hipLaunchKernelGGL((k_histo_apply_paged<CT, HOT, RUNS, false>), dim3((unsigned)groups),
                        dim3(1024), lds_apply, s, (const uint16_t*)staged, (const uint32_t*)tags,
                        (const uint32_t*)plist, (const uint32_t*)L.start, (const uint32_t*)L.segstart,
-                       L.misc, (int)B, seg_pages, tb, (uint64_t)nbins, P, P_cold, counts,
+                       L.misc, (int)B, tb, (uint64_t)nbins, P, P_cold, counts,
                        (const uint32_t*)flags);
   }
   return (int)hipGetLastError();

@@ -439,12 +439,23 @@ __device__ inline void part_scan_body(const uint32_t* __restrict__ count, int B,
   __syncthreads();
 }
 
+// misc[kSegWord]: the apply segment length k_part_scan chose (the last word
+// holds the sketch's flags).
+constexpr int kSegWord = kMiscWords - 2;
+
+// seg_alt, when nonzero, replaces seg_len if the sketch (flags) found no hot
+// bins: near-uniform buckets are about equally full, so segments of twice the
+// average leave every bucket to one workgroup (a plain-add flush), while
+// skewed inputs keep the shorter segments that spread their heavy buckets.
 [[maybe_unused]] static __global__ __launch_bounds__(1024) void k_part_scan(const uint32_t* __restrict__ count, int B,
-                                                           uint32_t seg_len,
+                                                           uint32_t seg_len, uint32_t seg_alt,
+                                                           const uint32_t* __restrict__ flags,
                                                            uint32_t* __restrict__ start,
                                                            uint32_t* __restrict__ segstart,
                                                            uint32_t* __restrict__ misc) {
-  part_scan_body(count, B, seg_len, start, segstart, misc);
+  const uint32_t seg = seg_alt && flags && !(*flags & kFlagHotBins) ? seg_alt : seg_len;
+  part_scan_body(count, B, seg, start, segstart, misc);
+  if (threadIdx.x == 0) misc[kSegWord] = seg;
 }
 
 // ---- pass 3: scatter through LDS send buffers -------------------------------------

@@ -427,8 +427,8 @@ int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const 
                        L.chunk);
     hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk, (int)C, B,
                        L.total);
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, B, 1u << 30, L.start,
-                       L.segstart, L.misc);
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, B, 1u << 30, 0u,
+                       (const uint32_t*)nullptr, L.start, L.segstart, L.misc);
     const bool last = k == passes - 1;
     void* out = last ? (void*)t_nonzero : (void*)buf[(k + 1) & 1];
     const size_t lds = deliver_lds<2>(B);

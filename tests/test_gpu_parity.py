@@ -146,6 +146,29 @@ def test_histo_bucket_pages(dev, orc, nbins, n, tile_log2, dist):
         assert np.array_equal(to_np(counts), want), (hot, merge)
 
 
+@pytest.mark.parametrize("nbins,n,dist", [
+    (1 << 17, 1 << 21, 0),           # narrowest planned buckets (2^10 bins)
+    ((1 << 20) + 3, 1 << 21, 0),     # ragged last bucket, uniform: whole-bucket segments
+    ((1 << 20) + 3, 1 << 21, 1),     # skewed: the sketch keeps the shorter segments
+    ((1 << 23) + 5, 1 << 22, 0),     # 2^15-bin buckets, fewer updates than bins
+])
+def test_histo_segment_choice(dev, orc, nbins, n, dist):
+    """The planned bucketed histogram picks its apply segment on the device
+    from the sketch (whole buckets without hot bins); a fixed segment length
+    (task_log2) and narrow buckets (about 2048 of them) give the same counts."""
+    idx = gen(dev, n, nbins, dist, 1.1, 0, seed=nbins + 7 * n)
+    want = np.zeros(nbins, np.int64)
+    orc.histo_agi(to_np(idx), want)
+    plan = B.histo_plan(n, nbins)
+    assert plan.strategy == B.BUCKET and plan.hot == 2
+    cfgs = [plan, B.config(B.BUCKET, block=1024, tile_log2=plan.tile_log2, hot=2, task_log2=14),
+            B.config(B.BUCKET, block=1024, tile_log2=max(9, (nbins - 1).bit_length() - 12), hot=2)]
+    for cfg in cfgs:
+        counts = torch.zeros(nbins, dtype=torch.int64, device=dev)
+        assert B.histo(idx, counts, cfg=cfg) == 0
+        assert np.array_equal(to_np(counts), want), cfg.as_dict()
+
+
 @pytest.mark.parametrize("distinct", [600, 1100])
 @pytest.mark.parametrize("dtype", [torch.int64, torch.int32])
 def test_histo_packed_tile_spill_table(dev, distinct, dtype):
