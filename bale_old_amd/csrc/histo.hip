@@ -1128,6 +1128,16 @@ static int64_t apply_segment(int64_t n, int tb, const bale_config& c, const Devi
   return std::max<int64_t>({4096, ceil_div(n, (int64_t)d->cus * 2), (int64_t)4 << tb});
 }
 
+// Apply segment of the adaptive plan for inputs with hot bins: about half the
+// average bucket (two segments for an average one, more for the heavy ones),
+// at most the default and at least 16384 updates (shorter segments cost more
+// in flushes than they gain in balance).
+static int64_t skewed_segment(int64_t n, int64_t B, int tb, const bale_config& c, const Device* d) {
+  const int64_t def = apply_segment(n, tb, c, d);
+  if (c.task_log2 > 0) return def;
+  return std::max<int64_t>(16384, std::min<int64_t>(def, ceil_div(n, 2 * B)));
+}
+
 // Apply segment for inputs the sketch finds without hot bins (taken on the
 // device): twice the average bucket, rounded up to a power of two, so nearly
 // every bucket is one segment.  0 (no alternative) when the caller fixed the
@@ -1175,7 +1185,7 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
                      (const uint32_t*)flags);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);
-  int64_t seg = apply_segment(n, tb, c, d);
+  int64_t seg = flags ? skewed_segment(n, B, tb, c, d) : apply_segment(n, tb, c, d);
   const int64_t seg_alt = uniform_segment(n, B, c);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, (int)B, (uint32_t)seg,
                      (uint32_t)seg_alt, (const uint32_t*)flags, L.start, L.segstart, L.misc);
@@ -1259,7 +1269,7 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk, (int)C, (int)B,
                      L.total);
   // apply segments in pages of the hot-on depth
-  const int64_t seg = apply_segment(n, tb, c, d);
+  const int64_t seg = adaptive ? skewed_segment(n, B, tb, c, d) : apply_segment(n, tb, c, d);
   const uint32_t seg_pages = (uint32_t)ceil_div(seg, P);
   // the uniform alternative is only taken without hot bins: cold pages
   const uint32_t seg_alt_pages =
