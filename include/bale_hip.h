@@ -207,9 +207,12 @@ typedef struct bale_config {
   int32_t hot;        /* histogram DIRECT/BUCKET: 1 = count the bins a 4096-index sample
                          sees twice in a per-workgroup LDS hash table; 0 = off;
                          BUCKET only: 2 = let that sample decide on the device, per
-                         call, whether to use the hot table and run merging */
+                         call, whether to use the hot table and run merging, and (with
+                         task_log2 = 0) the apply segment length */
   int32_t task_log2;  /* batches: operations per task (log2, 12..22); 0 = 16.
-                         BUCKET histograms: updates per apply segment (log2); 0 = by size */
+                         BUCKET histograms: updates per apply segment (log2); 0 = by size
+                         (with hot = 2: twice the average bucket when the sample finds
+                         no hot bins, half of it, at least 16384, when it does) */
 } bale_config;
 
 /* Select the device for this thread's later calls and create its state.
