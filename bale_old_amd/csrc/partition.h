@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void k_sketch_sample(const int64_t* __restrict
   scratch[kSketch + k] = code;
 }
 
-static __global__ __launch_bounds__(1024) void k_sketch_build(const uint32_t* __restrict__ scratch, uint64_t limit,
+[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_sketch_build(const uint32_t* __restrict__ scratch, uint64_t limit,
                                                        uint32_t* __restrict__ hot_out,
                                                        uint32_t* __restrict__ flags) {
   constexpr int kPer = kSketch / 1024;
