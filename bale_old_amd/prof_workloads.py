@@ -7,6 +7,8 @@
           calibration that converts the counters to bytes
   histo   the planned (bucketed) histogram of 2^28 uniform updates into 2^26 int64 bins
   zipf    the same plan on 2^28 Zipf(1.1) updates (its sample switches the hot table on)
+  small   the planned histogram of 2^24 uniform updates into 2^20 int64 bins
+          (8 MiB of counters, small enough to stay in the caches)
 """
 from __future__ import annotations
 
@@ -47,6 +49,12 @@ def main(argv=None) -> int:
         counts = torch.zeros(1 << 26, dtype=torch.int64, device=dev)
         B.gen_index(idx, 1 << 26, dist=1, s=1.1, seed=3)
         B.histo(idx, counts, cfg=B.histo_plan(n, 1 << 26))
+        del counts
+    if "small" in which:
+        m, nb = 1 << 24, 1 << 20
+        counts = torch.zeros(nb, dtype=torch.int64, device=dev)
+        B.gen_index(idx[:m], nb, seed=1)
+        B.histo(idx[:m], counts, cfg=B.histo_plan(m, nb))
         del counts
     torch.cuda.synchronize()
     print("prof_workloads done:", " ".join(sorted(which)), flush=True)
