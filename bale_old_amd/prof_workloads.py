@@ -9,6 +9,8 @@
   zipf    the same plan on 2^28 Zipf(1.1) updates (its sample switches the hot table on)
   small   the planned histogram of 2^24 uniform updates into 2^20 int64 bins
           (8 MiB of counters, small enough to stay in the caches)
+  batch   the held-out mix of 4096 instances (seed 5), strategies from the learned
+          selector, captured once and launched three times
 """
 from __future__ import annotations
 
@@ -56,6 +58,17 @@ def main(argv=None) -> int:
         B.gen_index(idx[:m], nb, seed=1)
         B.histo(idx[:m], counts, cfg=B.histo_plan(m, nb))
         del counts
+    if "batch" in which:
+        del idx
+        from bale_old_amd import mix as MX, selector as S
+        mix = MX.Mix(MX.mix_params(4096, 5), dev, st)
+        weights, _ = S.load(dev)
+        S.select(mix.inst, 4096, weights, dev, st)
+        b = MX.Batch(mix.inst, 4096, st)
+        for _ in range(3):
+            b.launch(st)
+        torch.cuda.synchronize()
+        b.close()
     torch.cuda.synchronize()
     print("prof_workloads done:", " ".join(sorted(which)), flush=True)
     return 0
