@@ -135,7 +135,9 @@ static void plan_gather(const Device*, int64_t, int64_t, bale_config* c) {
   memset(c, 0, sizeof *c);
   c->strategy = BALE_STRAT_DIRECT;
   c->block = 256;
-  c->unroll = 16;
+  // 8 table loads in flight per lane: 6.62 ms for 2^28 requests into a 2 GiB
+  // table on MI355X, against 6.79 ms with 16 and 6.73 ms with 4
+  c->unroll = 8;
   c->merge = BALE_MERGE_NONE;
 }
 
