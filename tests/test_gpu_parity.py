@@ -161,6 +161,8 @@ def test_histo_segment_choice(dev, orc, nbins, n, dist):
     orc.histo_agi(to_np(idx), want)
     plan = B.histo_plan(n, nbins)
     assert plan.strategy == B.BUCKET and plan.hot == 2
+    # about 256 buckets, 2^10..2^16 bins each
+    assert plan.tile_log2 == max(10, min(16, (nbins - 1).bit_length() - 8))
     cfgs = [plan, B.config(B.BUCKET, block=1024, tile_log2=plan.tile_log2, hot=2, task_log2=14),
             B.config(B.BUCKET, block=1024, tile_log2=max(9, (nbins - 1).bit_length() - 12), hot=2)]
     for cfg in cfgs:
