@@ -22,6 +22,7 @@
 
 #include "bale_internal.h"
 #include "partition.h"
+#include "pk_count.h"
 
 namespace bale {
 
@@ -216,81 +217,82 @@ __device__ inline void lds_add8(uint32_t* sh, u32x4 x) {
 }
 
 // Packed tile for 2^16-bin buckets: two 16-bit counters per LDS word (bin k in
-// half k & 1 of word k >> 1), 128 KiB for the whole bucket.  A counter never
-// wraps: an add that carries its counter across a multiple of kSpill moves
-// kSpill to the device counter and takes it off the LDS word.  With U the
-// counter's value less kSpill per correction still owed, every add keeps
-// U < kSpill, so a counter reaches 2^16 only if 64 of its corrections are
-// owed at once, i.e. about 2^16 - kSpill adds landed on that one bin (at most
-// 8 per LDS atomic, one LDS atomic per bank per cycle) between one lane's
-// atomic returning and that lane's correction: thousands of cycles of a
-// wave that is ready to issue.
-constexpr uint32_t kSpill = 1024;
-// The kSpill moved off an LDS word is counted in LDS too: a bit per bin says
-// the bin spilled, a small hash table (bin -> number of spills) counts them,
-// and the flush adds kSpill per spill.  A segment of m updates spills at most
-// m / kSpill times, so the table (kSpillSlots) fills only for segments of more
-// than kSpillSlots * kSpill updates; then the spill goes to the device counter
-// with an atomic, and `flag` makes the flush atomic as well.
-constexpr int kSpillBits = 10;
-constexpr uint32_t kSpillSlots = 1u << kSpillBits;
-struct PkSpill {
-  uint32_t* flag;  // some spill went to a device counter
+// half k & 1 of word k >> 1), 128 KiB for the whole bucket.  The words only
+// ever receive atomic adds; each lane reads from the value its own add returned
+// what that add owes the word's two bins when a half wrapped (pk_owed), so the
+// counts are exact whatever order the adds land in.  The owed amounts (2^16
+// per wrap, less one per carry the high half took from the low one) are
+// counted in LDS: a bit per bin says the bin is owed something and a small
+// hash table (bin -> owed, mod 2^32) holds the amounts.  A wrap takes 2^16
+// adds to one bin, so a segment of m updates notes at most m / 2^15 bins and
+// the table (kTallySlots) fills only for segments of more than 2^25 updates;
+// then the amount goes to the device counter with an atomic, and `flag` makes
+// the flush atomic as well.
+constexpr int kTallyBits = 10;
+constexpr uint32_t kTallySlots = 1u << kTallyBits;
+struct PkTally {
+  uint32_t* flag;  // some owed amount went to a device counter
   uint32_t* bits;  // [2^tb / 32]
-  uint32_t* keys;  // [kSpillSlots], kHotEmpty when free
-  uint32_t* cnts;  // [kSpillSlots]
+  uint32_t* keys;  // [kTallySlots], kHotEmpty when free
+  uint32_t* cnts;  // [kTallySlots] owed, mod 2^32
+  uint32_t width;  // bins of this tile
 };
-__device__ inline uint32_t spill_hash(uint32_t bin) { return (bin * 0x9E3779B1u) >> (32 - kSpillBits); }
-__device__ inline bool pk_spill_note(const PkSpill& sp, uint32_t bin) {
-  uint32_t h = spill_hash(bin);
-  for (uint32_t probe = 0; probe < kSpillSlots; probe++) {
-    const uint32_t prev = atomicCAS(&sp.keys[h], kHotEmpty, bin);
+__device__ inline uint32_t tally_hash(uint32_t bin) { return (bin * 0x9E3779B1u) >> (32 - kTallyBits); }
+__device__ inline bool pk_tally_note(const PkTally& t, uint32_t bin, uint32_t amount) {
+  uint32_t h = tally_hash(bin);
+  for (uint32_t probe = 0; probe < kTallySlots; probe++) {
+    const uint32_t prev = atomicCAS(&t.keys[h], kHotEmpty, bin);
     if (prev == kHotEmpty || prev == bin) {
-      atomicAdd(&sp.cnts[h], 1u);
-      atomicOr(&sp.bits[bin >> 5], 1u << (bin & 31u));
+      atomicAdd(&t.cnts[h], amount);
+      atomicOr(&t.bits[bin >> 5], 1u << (bin & 31u));
       return true;
     }
-    h = (h + 1u) & (kSpillSlots - 1u);
+    h = (h + 1u) & (kTallySlots - 1u);
   }
   return false;
 }
-// Spills recorded for a bin whose bit is set (after the segment's barrier).
-__device__ inline uint32_t pk_spill_count(const PkSpill& sp, uint32_t bin) {
-  uint32_t h = spill_hash(bin);
-  for (uint32_t probe = 0; probe < kSpillSlots; probe++) {
-    if (sp.keys[h] == bin) return sp.cnts[h];
-    h = (h + 1u) & (kSpillSlots - 1u);
+// What a bin whose bit is set is owed (after the segment's barrier).
+__device__ inline uint32_t pk_tally_of(const PkTally& t, uint32_t bin) {
+  uint32_t h = tally_hash(bin);
+  for (uint32_t probe = 0; probe < kTallySlots; probe++) {
+    if (t.keys[h] == bin) return t.cnts[h];
+    h = (h + 1u) & (kTallySlots - 1u);
   }
   return 0;
 }
-// LDS words after a packed tile of 2^tb bins: spill bits and table.
-__host__ __device__ inline uint32_t pk_spill_words(int tb) { return (1u << tb) / 32u + 2u * kSpillSlots; }
+// LDS words after a packed tile of 2^tb bins: tally bits and table.
+__host__ __device__ inline uint32_t pk_tally_words(int tb) { return (1u << tb) / 32u + 2u * kTallySlots; }
 
 template <typename CT>
-__device__ inline void pk_fix(uint32_t* sh, uint32_t bin, uint32_t inc, uint32_t old, CT* dst,
-                              const PkSpill& sp) {
-  const uint32_t sft = (bin & 1u) * 16u;
-  if ((((old >> sft) & 0xffffu) & (kSpill - 1u)) + inc >= kSpill) {
-    atomicAdd(&sh[bin >> 1], 0u - (kSpill << sft));
-    if (!pk_spill_note(sp, bin)) {
-      atomic_add_dev(dst + bin, (CT)kSpill);
-      *sp.flag = 1u;
-    }
+__device__ inline void pk_tally_add(const PkTally& t, uint32_t bin, uint32_t amount, CT* dst) {
+  if (!pk_tally_note(t, bin, amount)) {
+    atomic_add_dev(dst + bin, (CT)(int32_t)amount);
+    *t.flag = 1u;
   }
 }
-// Zero a packed tile's spill bits and table.
-__device__ inline void pk_spill_clear(const PkSpill& sp, int tb, int tid, int nthr) {
-  for (uint32_t k = tid; k < (1u << tb) / 32u; k += nthr) sp.bits[k] = 0;
-  for (uint32_t k = tid; k < kSpillSlots; k += nthr) {
-    sp.keys[k] = kHotEmpty;
-    sp.cnts[k] = 0;
+// Account for an add of inc to bin that returned old (pk_owed).  The high half
+// of a tile's last word has no bin when the width is odd: its carries are
+// never read.
+template <typename CT>
+__device__ inline void pk_account(uint32_t bin, uint32_t inc, uint32_t old, CT* dst,
+                                  const PkTally& t) {
+  const PkOwed o = pk_owed(old, bin & 1u, inc);
+  if (o.own) pk_tally_add(t, bin, o.own, dst);
+  if (o.high && (bin | 1u) < t.width) pk_tally_add(t, bin | 1u, o.high, dst);
+}
+// Zero a packed tile's tally bits and table.
+__device__ inline void pk_tally_clear(const PkTally& t, int tb, int tid, int nthr) {
+  for (uint32_t k = tid; k < (1u << tb) / 32u; k += nthr) t.bits[k] = 0;
+  for (uint32_t k = tid; k < kTallySlots; k += nthr) {
+    t.keys[k] = kHotEmpty;
+    t.cnts[k] = 0;
   }
 }
 
 // lds_add8 for the packed tile; all of a call's atomics are issued before
 // their results are checked.
 template <bool RUNS, typename CT>
-__device__ inline void lds_add8_pk(uint32_t* sh, u32x4 x, CT* dst, const PkSpill& spilled) {
+__device__ inline void lds_add8_pk(uint32_t* sh, u32x4 x, CT* dst, const PkTally& tally) {
   uint32_t v[8] = {x.x & 0xffff, x.x >> 16, x.y & 0xffff, x.y >> 16,
                    x.z & 0xffff, x.z >> 16, x.w & 0xffff, x.w >> 16};
   uint32_t inc[8], old[8];
@@ -310,13 +312,13 @@ __device__ inline void lds_add8_pk(uint32_t* sh, u32x4 x, CT* dst, const PkSpill
     old[i] = inc[i] ? atomicAdd(&sh[v[i] >> 1], inc[i] << ((v[i] & 1u) * 16u)) : 0u;
 #pragma unroll
   for (int i = 0; i < 8; i++)
-    if (inc[i]) pk_fix(sh, v[i], inc[i], old[i], dst, spilled);
+    if (inc[i]) pk_account(v[i], inc[i], old[i], dst, tally);
 }
 
 template <typename CT>
-__device__ inline void lds_add1_pk(uint32_t* sh, uint32_t bin, CT* dst, const PkSpill& spilled) {
+__device__ inline void lds_add1_pk(uint32_t* sh, uint32_t bin, CT* dst, const PkTally& tally) {
   const uint32_t old = atomicAdd(&sh[bin >> 1], 1u << ((bin & 1u) * 16u));
-  pk_fix(sh, bin, 1u, old, dst, spilled);
+  pk_account(bin, 1u, old, dst, tally);
 }
 
 // Add an LDS tile of counts (32-bit, or PK: two 16-bit per word) into
@@ -326,11 +328,10 @@ __device__ inline void lds_add1_pk(uint32_t* sh, uint32_t bin, CT* dst, const Pk
 // streams instead of waiting on one load at a time.
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 template <bool PK>
-__device__ inline uint32_t tile_count(const uint32_t* sh, uint32_t k, const PkSpill& sp) {
+__device__ inline uint32_t tile_count(const uint32_t* sh, uint32_t k, const PkTally& t) {
   if (!PK) return sh[k];
-  uint32_t c = (sh[k >> 1] >> ((k & 1u) * 16u)) & 0xffffu;
-  if ((sp.bits[k >> 5] >> (k & 31u)) & 1u) c += kSpill * pk_spill_count(sp, k);
-  return c;
+  const uint32_t owed = ((t.bits[k >> 5] >> (k & 31u)) & 1u) ? pk_tally_of(t, k) : 0u;
+  return pk_read(sh[k >> 1], k & 1u, owed);
 }
 // four consecutive counters as 16-byte vectors
 struct Cnt4_64 { i64x2 a, b; };
@@ -347,7 +348,7 @@ __device__ inline void store4(int32_t* p, i32x4 v, const uint32_t* c) {
 }
 template <typename CT, bool PK>
 __device__ inline void flush_tile(const uint32_t* sh, uint32_t width, CT* dst, bool plain, int tid,
-                                  int nthr, const PkSpill& sp) {
+                                  int nthr, const PkTally& sp) {
   if (!plain) {
     for (uint32_t k = tid; k < width; k += nthr) {
       const uint32_t c = tile_count<PK>(sh, k, sp);
@@ -397,11 +398,9 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
                                                       const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
   __shared__ uint32_t s_work[4];
-  __shared__ uint32_t s_spilled;  // PK: some spill of this segment went to a device counter
-  // PK: spill bits and table after the tile (2^(shift-1) words)
-  uint32_t* const spill_base = sh + (PK ? 1u << (shift - 1) : 0u);
-  const PkSpill sp{&s_spilled, spill_base, spill_base + (1u << shift) / 32u,
-                   spill_base + (1u << shift) / 32u + kSpillSlots};
+  __shared__ uint32_t s_owed_dev;  // PK: some owed amount of this segment went to a device counter
+  // PK: tally bits and table after the tile (2^(shift-1) words)
+  uint32_t* const tally_base = sh + (PK ? 1u << (shift - 1) : 0u);
   const bool merge = RUNS && (!flags || (*flags & kFlagRunsSeen));
   const int tid = threadIdx.x, nthr = blockDim.x;
   const uint32_t total = misc[0], seg_len = misc[kSegWord];
@@ -420,7 +419,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
       }
       s_work[0] = w;
       s_work[1] = b;
-      s_spilled = 0;
+      s_owed_dev = 0;
     }
     __syncthreads();
     const uint32_t w = s_work[0], b = s_work[1];
@@ -434,7 +433,9 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
     const uint32_t words = PK ? (width + 1u) >> 1 : width;
     CT* dst = counts + lo_bin;
     for (uint32_t k = tid; k < words; k += nthr) sh[k] = 0;
-    if (PK) pk_spill_clear(sp, shift, tid, nthr);
+    const PkTally sp{&s_owed_dev, tally_base, tally_base + (1u << shift) / 32u,
+                     tally_base + (1u << shift) / 32u + kTallySlots, width};
+    if (PK) pk_tally_clear(sp, shift, tid, nthr);
     __syncthreads();
     // unaligned head, then 8 entries per 16-byte load
     uint32_t a0 = min(e1, (e0 + 7u) & ~7u);
@@ -460,8 +461,8 @@ __global__ __launch_bounds__(1024) void k_histo_apply(const uint16_t* __restrict
     }
     __syncthreads();
     // a bucket with one segment is this workgroup's alone: plain adds, unless
-    // a spill already went to its counters atomically
-    const bool plain = nseg == 1 && !(PK && s_spilled);
+    // an owed amount already went to its counters atomically
+    const bool plain = nseg == 1 && !(PK && s_owed_dev);
     flush_tile<CT, PK>(sh, width, dst, plain, tid, nthr, sp);
     __syncthreads();
   }
@@ -877,7 +878,7 @@ __global__ __launch_bounds__(1024) void k_page_list(const uint32_t* __restrict__
 // lds_add8 / lds_add8_pk for the first `valid` (1..8) of the 8 offsets in x.
 template <bool RUNS, bool PK, typename CT>
 __device__ inline void lds_add_valid(uint32_t* sh, u32x4 x, uint32_t valid, CT* dst,
-                                     const PkSpill& spilled) {
+                                     const PkTally& tally) {
   uint32_t v[8] = {x.x & 0xffff, x.x >> 16, x.y & 0xffff, x.y >> 16,
                    x.z & 0xffff, x.z >> 16, x.w & 0xffff, x.w >> 16};
   uint32_t inc[8], old[8];
@@ -903,7 +904,7 @@ __device__ inline void lds_add_valid(uint32_t* sh, u32x4 x, uint32_t valid, CT* 
     old[i] = inc[i] ? atomicAdd(&sh[v[i] >> 1], inc[i] << ((v[i] & 1u) * 16u)) : 0u;
 #pragma unroll
   for (int i = 0; i < 8; i++)
-    if (inc[i]) pk_fix(sh, v[i], inc[i], old[i], dst, spilled);
+    if (inc[i]) pk_account(v[i], inc[i], old[i], dst, tally);
 }
 
 // Persistent: each workgroup takes segments (<= seg_pages pages of one bucket)
@@ -918,11 +919,9 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     const uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t sh[];
   __shared__ uint32_t s_work[2];
-  __shared__ uint32_t s_spilled;  // PK: some spill of this segment went to a device counter
-  // PK: spill bits and table after the tile (2^(shift-1) words)
-  uint32_t* const spill_base = sh + (PK ? 1u << (shift - 1) : 0u);
-  const PkSpill sp{&s_spilled, spill_base, spill_base + (1u << shift) / 32u,
-                   spill_base + (1u << shift) / 32u + kSpillSlots};
+  __shared__ uint32_t s_owed_dev;  // PK: some owed amount of this segment went to a device counter
+  // PK: tally bits and table after the tile (2^(shift-1) words)
+  uint32_t* const tally_base = sh + (PK ? 1u << (shift - 1) : 0u);
   const bool merge = RUNS && (!flags || (*flags & kFlagRunsSeen));
   const uint32_t Se = (uint32_t)paged_depth<HOT>(flags, S, S_cold);
   const uint32_t vpp = Se / 8u;  // 16-byte vectors per page (a power of two)
@@ -943,7 +942,7 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
       }
       s_work[0] = w;
       s_work[1] = b;
-      s_spilled = 0;
+      s_owed_dev = 0;
     }
     __syncthreads();
     const uint32_t w = s_work[0], b = s_work[1];
@@ -956,7 +955,9 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     const uint32_t words = PK ? (width + 1u) >> 1 : width;
     CT* dst = counts + lo_bin;
     for (uint32_t k = tid; k < words; k += nthr) sh[k] = 0;
-    if (PK) pk_spill_clear(sp, shift, tid, nthr);
+    const PkTally sp{&s_owed_dev, tally_base, tally_base + (1u << shift) / 32u,
+                     tally_base + (1u << shift) / 32u + kTallySlots, width};
+    if (PK) pk_tally_clear(sp, shift, tid, nthr);
     __syncthreads();
     const uint32_t nv = (g1 - g0) << vsh;
     constexpr int Q = 4;  // vectors per thread in flight
@@ -992,8 +993,8 @@ __global__ __launch_bounds__(1024) void k_histo_apply_paged(
     }
     __syncthreads();
     // a bucket with one segment is this workgroup's alone: plain adds, unless
-    // a spill already went to its counters atomically
-    const bool plain = nseg == 1 && !(PK && s_spilled);
+    // an owed amount already went to its counters atomically
+    const bool plain = nseg == 1 && !(PK && s_owed_dev);
     flush_tile<CT, PK>(sh, width, dst, plain, tid, nthr, sp);
     __syncthreads();
   }
@@ -1201,7 +1202,7 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   hipLaunchKernelGGL((k_histo_scatter<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024), lds, s, idx,
                      n, (uint64_t)nbins, tb, (int)B, S, chunk, L.chunk, L.start, L.hot,
                      (uint16_t*)L.staged, counts, (const uint32_t*)flags, S_cold);
-  size_t lds_apply = ((size_t)(pk ? 2 : 4) << tb) + (pk ? (size_t)pk_spill_words(tb) * 4 : 0);
+  size_t lds_apply = ((size_t)(pk ? 2 : 4) << tb) + (pk ? (size_t)pk_tally_words(tb) * 4 : 0);
   int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
   if (pk) {
     allow_lds(k_histo_apply<CT, RUNS, true>, lds_apply);
@@ -1280,7 +1281,7 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   hipLaunchKernelGGL(k_page_list<HOT>, dim3((unsigned)C), dim3(1024), page_list_lds((int)B), s,
                      (const uint32_t*)tags, (const uint32_t*)L.chunk, (const uint32_t*)L.start,
                      (const uint32_t*)used, (int)B, chunk, P, P_cold, (const uint32_t*)flags, plist);
-  size_t lds_apply = ((size_t)(pk ? 2 : 4) << tb) + (pk ? (size_t)pk_spill_words(tb) * 4 : 0);
+  size_t lds_apply = ((size_t)(pk ? 2 : 4) << tb) + (pk ? (size_t)pk_tally_words(tb) * 4 : 0);
   int64_t groups = (int64_t)d->cus * (lds_apply <= 65536 ? 2 : 1);
   if (pk) {
     allow_lds(k_histo_apply_paged<CT, HOT, RUNS, true>, lds_apply);

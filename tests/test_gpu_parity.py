@@ -102,15 +102,25 @@ def test_histo_tiles(dev, orc, tile_log2):
         assert np.array_equal(to_np(counts), want), (strat, hot)
 
 
-@pytest.mark.parametrize("case", ["one_bin", "zipf1.4", "runs64"])
+@pytest.mark.parametrize("case", ["one_bin", "word_pair", "zipf1.4", "runs64"])
 @pytest.mark.parametrize("dtype", [torch.int64, torch.int32])
 def test_histo_packed_tile_spills(dev, case, dtype):
-    """2^16-bin buckets use 16-bit LDS counters that spill every 1024 adds:
-    drive single bins far past 2^16 adds within one bucket segment."""
+    """2^16-bin buckets keep two 16-bit LDS counters per word; a counter that
+    wraps is owed 2^16 and a low-half wrap's carry is taken off the high bin:
+    drive single bins, and both bins of one word, far past 2^16 adds within one
+    bucket segment, and the last bin of an odd-width bucket."""
     if case == "one_bin":
         nbins, n = (1 << 17) + 3, 1 << 21
         idx = torch.full((n,), 70001, dtype=torch.int64, device=dev)
         idx[::3] = 131074  # second bucket, last (partial) tile
+    elif case == "word_pair":
+        # bins 70000 / 70001 share a word, 131074 is the odd-width tile's
+        # last bin (low half; its high half is no bin)
+        nbins, n = (1 << 17) + 3, 3 << 20
+        idx = torch.full((n,), 70000, dtype=torch.int64, device=dev)
+        idx[1::3] = 70001
+        idx[2::6] = 131074
+        idx[5::12] = 131073
     elif case == "zipf1.4":
         nbins, n = (1 << 18) + 5, 1 << 21
         idx = gen(dev, n, nbins, 1, 1.4, 0, seed=17)
@@ -150,7 +160,8 @@ def test_histo_bucket_pages(dev, orc, nbins, n, tile_log2, dist):
     (1 << 17, 1 << 21, 0),           # narrowest planned buckets (2^10 bins)
     ((1 << 20) + 3, 1 << 21, 0),     # ragged last bucket, uniform: whole-bucket segments
     ((1 << 20) + 3, 1 << 21, 1),     # skewed: the sketch keeps the shorter segments
-    ((1 << 23) + 5, 1 << 22, 0),     # 2^15-bin buckets, fewer updates than bins
+    ((1 << 23) + 5, 1 << 22, 0),     # 2^16-bin (packed 16-bit) buckets, fewer updates than bins
+    ((1 << 22) + 5, 1 << 22, 1),     # 2^15-bin (32-bit) buckets, skewed
 ])
 def test_histo_segment_choice(dev, orc, nbins, n, dist):
     """The planned bucketed histogram picks its apply segment on the device
@@ -171,19 +182,20 @@ def test_histo_segment_choice(dev, orc, nbins, n, dist):
         assert np.array_equal(to_np(counts), want), cfg.as_dict()
 
 
-@pytest.mark.parametrize("distinct", [600, 1100])
+@pytest.mark.parametrize("distinct", [300, 1100])
 @pytest.mark.parametrize("dtype", [torch.int64, torch.int32])
-def test_histo_packed_tile_spill_table(dev, distinct, dtype):
-    """Spills of the packed 16-bit tile are counted in an LDS table of 1024
-    bins; one 2^21-update segment with 1100 bins that each spill overflows it,
-    and the rest go to device atomics (the flush then turns atomic too)."""
-    nbins, reps = 1 << 16, 1100
+def test_histo_packed_tile_tally_table(dev, distinct, dtype):
+    """What wrapped packed counters are owed is kept in an LDS table of 1024
+    bins; one segment of 1100 bins that each wrap (each wrap of a low bin
+    also notes its high neighbour) overflows it, and the rest go to device
+    atomics (the flush then turns atomic too); 300 such bins fit."""
+    nbins, reps = 1 << 16, (1 << 16) + 100
     bins = (torch.arange(distinct, device=dev, dtype=torch.int64) * 59) % nbins
     idx = bins.repeat(reps)
     idx = idx[torch.randperm(idx.numel(), device=dev)]
     want = np.bincount(to_np(idx), minlength=nbins) + 3
     counts = torch.full((nbins,), 3, dtype=dtype, device=dev)
-    cfg = B.config(B.BUCKET, block=1024, tile_log2=16, hot=0, task_log2=21)
+    cfg = B.config(B.BUCKET, block=1024, tile_log2=16, hot=0, task_log2=27)
     assert B.histo(idx, counts, cfg=cfg) == 0
     assert np.array_equal(to_np(counts).astype(np.int64), want)
 
