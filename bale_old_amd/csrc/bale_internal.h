@@ -78,22 +78,44 @@ inline void allow_lds(K kernel, size_t bytes) {
                               (int)bytes);
 }
 
-// ---- per-device state (capi.hip) -----------------------------------------
+// ---- per-device and per-stream state (capi.hip) ----------------------------
+// Each stream a caller uses gets its own error record and workspace: a
+// stream's calls are ordered by the stream, so its workspace can grow (after
+// waiting for that stream alone) without freeing memory another stream's
+// kernels still use.
+struct StreamCtx {
+  bool used = false;
+  hipStream_t s = nullptr;
+  ErrWords* err = nullptr;  // index errors of the calls on this stream
+  void* ws = nullptr;       // grow-only workspace of the calls on this stream
+  size_t ws_bytes = 0;
+};
+constexpr int kMaxStreamCtx = 256;
 struct Device {
   int id = -1;
   int cus = 0;
   int lds_per_block = 0;
-  ErrWords* err = nullptr;    // device error record
-  void* ws = nullptr;         // grow-only workspace
-  size_t ws_bytes = 0;
+  StreamCtx ctx[kMaxStreamCtx];
 };
-Device* device_state();  // current device's state (created on first use)
+Device* device_state();  // current device's state (created on first use; allocates nothing)
+// The stream's error record and workspace holder, created on the stream's
+// first use (allocates the 16-byte record; refused while the stream is being
+// captured).  nullptr on failure, with the error text set.
+StreamCtx* stream_ctx(Device* d, hipStream_t s);
+// The error record kernels launched on s report to (nullptr if stream_ctx failed).
+ErrWords* stream_err(Device* d, hipStream_t s);
 // Record the calling thread's error text (bale_hip_last_error) and return code.
 int bale_set_error(int code, const char* fmt, ...);
-// Workspace of at least `bytes`, 256-byte aligned; nullptr on failure.
+// Workspace of at least `bytes` for kernels on s, 256-byte aligned; nullptr on
+// failure.
 void* workspace(Device* d, size_t bytes, hipStream_t s);
 // BALE_ERR_HIP with the runtime's text for rc (bale_hip_last_error).
 int bale_fail_hip(int rc, const char* what);
+// What a launcher returns for a configuration that cannot run on this shape
+// (the reason is already in the thread's error text); the API turns it into
+// BALE_ERR_ARG.  No HIP error code has this value.
+constexpr int kBadConfig = 0x7fff0001;
+int bad_config(const char* fmt, ...);
 // The calling thread's bale_hip_last_timing values.
 void set_last_timing(double gpu_s, double h2d_s, double d2h_s);
 double last_gpu_seconds();

@@ -38,6 +38,9 @@ enum : uint32_t {
   OP_GLDS = 7,
 };
 constexpr uint32_t kFlagHot = 1, kFlagRuns = 2;
+// the instance's counts overlap another histogram instance's counts in the
+// same batch: phase D flushes even a single-segment bucket with atomics
+constexpr uint32_t kFlagShared = 4;
 constexpr int kBlockA = 512;
 constexpr int kLdsAWords = 8192;  // 32 KiB: LDS tiles of up to 2^13 bins
 constexpr int kLdsTileBits = 13;
@@ -420,7 +423,8 @@ __global__ __launch_bounds__(1024) void k_batch_d(const BInst* __restrict__ inst
       }
     }
     __syncthreads();
-    if (nseg == 1) {
+    if (nseg == 1 && !(I.flags & kFlagShared)) {
+      // the bucket is this task's alone (no other instance writes these counts)
       for (uint32_t k = tid; k < width; k += T) {
         uint32_t c = sh[k];
         if (!c) continue;
@@ -434,6 +438,17 @@ __global__ __launch_bounds__(1024) void k_batch_d(const BInst* __restrict__ inst
       }
     }
     __syncthreads();
+  }
+}
+
+// Add the graph's error counts to a stream's record and clear them.
+__global__ void k_batch_err_move(ErrWords* __restrict__ from, ErrWords* __restrict__ to) {
+  if (threadIdx.x == 0) {
+    const unsigned long long r = from->range, p = from->pe;
+    if (r) atomicAdd(&to->range, r);
+    if (p) atomicAdd(&to->pe, p);
+    from->range = 0;
+    from->pe = 0;
   }
 }
 
@@ -490,6 +505,7 @@ struct Plan {
 
 struct Buffers {
   DevBuf arena, inst, a, qoff, c, d, hot_list, bucket_list, heads;
+  DevBuf err;  // the captured graph's index errors, moved to the launching stream's record
 };
 
 // Default per-instance rule when cfg.strategy is AUTO.
@@ -655,6 +671,8 @@ int plan_upload(Plan& P, Buffers& Bf, hipStream_t s) {
   if ((rc = up(Bf.hot_list, P.hot_list.data(), P.hot_list.size() * 4))) return rc;
   if ((rc = up(Bf.bucket_list, P.bucket_list.data(), P.bucket_list.size() * 4))) return rc;
   if ((rc = Bf.heads.ensure(64))) return rc;
+  if ((rc = Bf.err.ensure(sizeof(ErrWords)))) return rc;
+  if ((rc = (int)hipMemsetAsync(Bf.err.p, 0, sizeof(ErrWords), s))) return rc;
   return (int)hipStreamSynchronize(s);
 }
 
@@ -670,7 +688,7 @@ void set_batch_lds(const Plan& P) {
 }
 
 // Enqueue the phases on s (also used inside graph capture).
-void plan_enqueue(const Plan& P, Buffers& Bf, Device* d, hipStream_t s) {
+void plan_enqueue(const Plan& P, Buffers& Bf, Device* d, hipStream_t s, ErrWords* err) {
   const BInst* insts = (const BInst*)Bf.inst.p;
   uint32_t* heads = (uint32_t*)Bf.heads.p;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, heads, 4 + kXcds);
@@ -680,7 +698,7 @@ void plan_enqueue(const Plan& P, Buffers& Bf, Device* d, hipStream_t s) {
   if (!P.a.empty()) {
     int64_t g = std::min<int64_t>((int64_t)d->cus * 4, (int64_t)P.a.size());
     hipLaunchKernelGGL(k_batch_a, dim3((unsigned)g), dim3(kBlockA), (size_t)kLdsAWords * 4, s, insts,
-                       (const BTask*)Bf.a.p, (const uint32_t*)Bf.qoff.p, (uint32_t)kXcds, heads + 4, d->err);
+                       (const BTask*)Bf.a.p, (const uint32_t*)Bf.qoff.p, (uint32_t)kXcds, heads + 4, err);
   }
   if (!P.bucket_list.empty()) {
     hipLaunchKernelGGL(k_batch_b, dim3((unsigned)P.bucket_list.size()), dim3(256), 0, s, insts,
@@ -691,6 +709,37 @@ void plan_enqueue(const Plan& P, Buffers& Bf, Device* d, hipStream_t s) {
     int64_t gd = std::min<int64_t>((int64_t)d->cus, (int64_t)P.d.size());
     hipLaunchKernelGGL(k_batch_d, dim3((unsigned)gd), dim3(1024), kLdsD, s, insts,
                        (const BTask*)Bf.d.p, (uint32_t)P.d.size(), heads + 2);
+  }
+}
+
+// Histogram instances whose counts share bytes with another histogram
+// instance's counts (many index streams into one table is a natural batch)
+// get kFlagShared: their buckets' flushes race with the other instance's.
+void mark_shared_counts(Plan& P) {
+  struct Span { uintptr_t lo, hi; size_t id; };
+  std::vector<Span> v;
+  for (size_t i = 0; i < P.inst.size(); i++) {
+    const BInst& I = P.inst[i];
+    if (I.kind != 0 || I.n == 0) continue;
+    const uintptr_t lo = (uintptr_t)I.table;
+    v.push_back({lo, lo + (uintptr_t)I.table_len * (I.dtype == BALE_I64 ? 8u : 4u), i});
+  }
+  std::sort(v.begin(), v.end(), [](const Span& a, const Span& b) { return a.lo < b.lo; });
+  // clusters of spans chained by overlaps: every member of a cluster of two
+  // or more is marked (a plain flush must not meet an atomic one either)
+  size_t first = 0;
+  uintptr_t reach = 0;
+  for (size_t k = 0; k <= v.size(); k++) {
+    if (k < v.size() && k > first && v[k].lo < reach) {
+      reach = std::max(reach, v[k].hi);
+      continue;
+    }
+    if (k > first + 1)
+      for (size_t m = first; m < k; m++) P.inst[v[m].id].flags |= kFlagShared;
+    if (k < v.size()) {
+      first = k;
+      reach = v[k].hi;
+    }
   }
 }
 
@@ -754,6 +803,7 @@ bale_batch* bale_batch_create(const bale_instance* inst, int64_t count, void* st
       return nullptr;
     }
   }
+  mark_shared_counts(b->plan);
   hipStream_t us = (hipStream_t)stream;
   int rc = plan_upload(b->plan, b->bufs, us);
   if (rc) {
@@ -770,7 +820,7 @@ bale_batch* bale_batch_create(const bale_instance* inst, int64_t count, void* st
     delete b;
     return nullptr;
   }
-  plan_enqueue(b->plan, b->bufs, d, b->capture);
+  plan_enqueue(b->plan, b->bufs, d, b->capture, (ErrWords*)b->bufs.err.p);
   if (hipStreamEndCapture(b->capture, &b->graph) != hipSuccess ||
       hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
     (void)hipGetLastError();
@@ -783,8 +833,17 @@ bale_batch* bale_batch_create(const bale_instance* inst, int64_t count, void* st
 
 int bale_batch_launch(bale_batch* b, void* stream) {
   if (!b || !b->exec) return bale_set_error(BALE_ERR_ARG, "null batch");
-  int rc = (int)hipGraphLaunch(b->exec, (hipStream_t)stream);
-  return rc ? bale_set_error(BALE_ERR_HIP, "graph launch: %s", hipGetErrorString((hipError_t)rc))
+  Device* d = device_state();
+  if (!d) return bale_set_error(BALE_ERR_NODEV, "no usable HIP device");
+  hipStream_t s = (hipStream_t)stream;
+  ErrWords* err = stream_err(d, s);
+  if (!err) return BALE_ERR_ARG;
+  int rc = (int)hipGraphLaunch(b->exec, s);
+  if (rc) return bale_set_error(BALE_ERR_HIP, "graph launch: %s", hipGetErrorString((hipError_t)rc));
+  // the graph counted into the batch's own record; report on this stream's
+  hipLaunchKernelGGL(k_batch_err_move, dim3(1), dim3(64), 0, s, (ErrWords*)b->bufs.err.p, err);
+  rc = (int)hipGetLastError();
+  return rc ? bale_set_error(BALE_ERR_HIP, "batch error record: %s", hipGetErrorString((hipError_t)rc))
             : BALE_OK;
 }
 
@@ -792,7 +851,9 @@ int bale_batch_launch_direct(bale_batch* b, void* stream) {
   if (!b) return bale_set_error(BALE_ERR_ARG, "null batch");
   Device* d = device_state();
   if (!d) return bale_set_error(BALE_ERR_NODEV, "no usable HIP device");
-  plan_enqueue(b->plan, b->bufs, d, (hipStream_t)stream);
+  ErrWords* err = stream_err(d, (hipStream_t)stream);
+  if (!err) return BALE_ERR_ARG;
+  plan_enqueue(b->plan, b->bufs, d, (hipStream_t)stream, err);
   int rc = (int)hipGetLastError();
   return rc ? bale_set_error(BALE_ERR_HIP, "batch launch: %s", hipGetErrorString((hipError_t)rc))
             : BALE_OK;
@@ -876,6 +937,8 @@ int bale_tune(bale_instance* inst, int64_t count, int64_t min_ops, float* costs,
   Device* d = device_state();
   if (!d) return bale_set_error(BALE_ERR_NODEV, "no usable HIP device");
   hipStream_t s = (hipStream_t)stream;
+  ErrWords* err = stream_err(d, s);
+  if (!err) return BALE_ERR_ARG;
   if (min_ops <= 0) min_ops = (int64_t)1 << 22;
   Buffers bufs;
   DevBuf scratch;
@@ -915,9 +978,9 @@ int bale_tune(bale_instance* inst, int64_t count, int64_t min_ops, float* costs,
         return -2.f;
       }
       set_batch_lds(P);
-      plan_enqueue(P, bufs, d, s);  // warm-up
+      plan_enqueue(P, bufs, d, s, err);  // warm-up
       (void)hipEventRecord(e0, s);
-      plan_enqueue(P, bufs, d, s);
+      plan_enqueue(P, bufs, d, s, err);
       (void)hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) {
         rc = bale_set_error(BALE_ERR_HIP, "tuner run failed");

@@ -15,6 +15,8 @@
 namespace bale {
 
 static thread_local std::string t_err;
+static thread_local int t_code = BALE_OK;  // code of the last recorded error
+static thread_local int t_strategy = BALE_STRAT_AUTO;  // strategy of the last histo / gather
 static thread_local double t_gpu_s = 0.0, t_h2d_s = 0.0, t_d2h_s = 0.0;
 
 static int fail(int code, const char* fmt, ...) {
@@ -24,6 +26,7 @@ static int fail(int code, const char* fmt, ...) {
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
   t_err = buf;
+  t_code = code;
   return code;
 }
 int bale_set_error(int code, const char* fmt, ...) {
@@ -33,7 +36,19 @@ int bale_set_error(int code, const char* fmt, ...) {
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
   t_err = buf;
+  t_code = code;
   return code;
+}
+
+int bad_config(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  t_code = BALE_ERR_ARG;
+  return kBadConfig;
 }
 
 static int fail_hip(int rc, const char* what) {
@@ -68,31 +83,120 @@ Device* device_state() {
     lds = optin;
   (void)hipGetLastError();
   d->lds_per_block = lds;
-  if (hipMalloc((void**)&d->err, sizeof(ErrWords)) != hipSuccess) return nullptr;
-  if (hipMemset(d->err, 0, sizeof(ErrWords)) != hipSuccess) return nullptr;
-  if (hipDeviceSynchronize() != hipSuccess) return nullptr;
   d->id = id;
   return d;
 }
 
-void* workspace(Device* d, size_t bytes, hipStream_t s) {
-  if (d->ws_bytes >= bytes) return d->ws;
-  std::lock_guard<std::mutex> lk(g_mu);
-  if (d->ws_bytes >= bytes) return d->ws;
-  if (d->ws) {
-    if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
-    (void)hipFree(d->ws);
-    d->ws = nullptr;
-    d->ws_bytes = 0;
-  }
-  size_t want = bytes + (bytes >> 3);  // grow with headroom
-  if (hipMalloc(&d->ws, want) != hipSuccess) {
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) {
     (void)hipGetLastError();
-    d->ws = nullptr;
+    return false;
+  }
+  return st != hipStreamCaptureStatusNone;
+}
+
+StreamCtx* stream_ctx(Device* d, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  StreamCtx* free_slot = nullptr;
+  for (StreamCtx& c : d->ctx) {
+    if (c.used && c.s == s) return &c;
+    if (!c.used && !free_slot) free_slot = &c;
+  }
+  if (!free_slot) {
+    fail(BALE_ERR_ARG, "more than %d streams in use on device %d (bale_hip_release_stream frees one)",
+         kMaxStreamCtx, d->id);
     return nullptr;
   }
-  d->ws_bytes = want;
-  return d->ws;
+  if (capturing(s)) {
+    fail(BALE_ERR_ARG, "first call on a stream while it is being captured: make one call on it "
+                       "outside capture first (or use bale_histo_noalloc)");
+    return nullptr;
+  }
+  ErrWords* err = nullptr;
+  if (hipMalloc((void**)&err, sizeof(ErrWords)) != hipSuccess) {
+    (void)hipGetLastError();
+    fail(BALE_ERR_ALLOC, "error record allocation failed");
+    return nullptr;
+  }
+  if (hipMemsetAsync(err, 0, sizeof(ErrWords), s) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(err);
+    fail(BALE_ERR_HIP, "clearing the error record failed");
+    return nullptr;
+  }
+  free_slot->s = s;
+  free_slot->err = err;
+  free_slot->ws = nullptr;
+  free_slot->ws_bytes = 0;
+  free_slot->used = true;
+  return free_slot;
+}
+
+ErrWords* stream_err(Device* d, hipStream_t s) {
+  StreamCtx* c = stream_ctx(d, s);
+  return c ? c->err : nullptr;
+}
+
+// BALE_OK once the stream has its error record, else the error stream_ctx gave.
+static int stream_ready(Device* d, hipStream_t s) { return stream_ctx(d, s) ? BALE_OK : t_code; }
+
+// Launcher result to an API code: 0, a configuration the shape cannot take
+// (reason already recorded), allocation failure, or another HIP error.
+static int launch_result(int rc, const char* what) {
+  if (rc == 0) return BALE_OK;
+  if (rc == kBadConfig) return BALE_ERR_ARG;
+  if (rc == hipErrorOutOfMemory) return t_code == BALE_ERR_ALLOC ? BALE_ERR_ALLOC : fail(BALE_ERR_ALLOC, "%s: workspace allocation failed", what);
+  return fail_hip(rc, what);
+}
+
+// Fields of a caller's configuration that no shape can take.
+static int check_config(const bale_config& c, bool histogram) {
+  if (c.strategy < BALE_STRAT_AUTO || c.strategy > BALE_STRAT_BUCKET ||
+      (!histogram && c.strategy == BALE_STRAT_LDS))
+    return fail(BALE_ERR_ARG, "bad %s strategy %d", histogram ? "histogram" : "gather", c.strategy);
+  if (c.block != 0 && c.block != 256 && c.block != 512 && c.block != 1024)
+    return fail(BALE_ERR_ARG, "block %d: 0, 256, 512 or 1024 threads", c.block);
+  if (c.unroll != 0 && c.unroll != 4 && c.unroll != 8 && c.unroll != 16)
+    return fail(BALE_ERR_ARG, "unroll %d: 0, 4, 8 or 16", c.unroll);
+  if (c.tile_log2 < 0 || c.tile_log2 > 30)
+    return fail(BALE_ERR_ARG, "tile_log2 %d: 0 (default) or 1..30", c.tile_log2);
+  if (c.merge < -1 || c.merge > BALE_MERGE_LEADER)
+    return fail(BALE_ERR_ARG, "merge %d: -1 (default), 0, 1 or 2", c.merge);
+  if (c.groups_per_cu < 0 || c.groups_per_cu > 16)
+    return fail(BALE_ERR_ARG, "groups_per_cu %d: 0 (default) or 1..16", c.groups_per_cu);
+  if (c.hot < 0 || c.hot > 2 || (c.hot == 2 && c.strategy != BALE_STRAT_BUCKET && c.strategy != BALE_STRAT_AUTO))
+    return fail(BALE_ERR_ARG, "hot %d: 0, 1, or 2 with the BUCKET strategy", c.hot);
+  if (c.task_log2 != 0 && (c.task_log2 < 12 || c.task_log2 > 30))
+    return fail(BALE_ERR_ARG, "task_log2 %d: 0 (default) or 12..30", c.task_log2);
+  return BALE_OK;
+}
+
+void* workspace(Device* d, size_t bytes, hipStream_t s) {
+  StreamCtx* c = stream_ctx(d, s);
+  if (!c) return nullptr;
+  if (c->ws_bytes >= bytes) return c->ws;
+  if (capturing(s)) {
+    fail(BALE_ERR_ARG, "the stream's workspace must grow to %zu bytes, which cannot happen during "
+                       "capture: make the same call once outside capture first", bytes);
+    return nullptr;
+  }
+  if (c->ws) {
+    // only this stream's kernels use it
+    if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
+    (void)hipFree(c->ws);
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+  }
+  size_t want = bytes + (bytes >> 3);  // grow with headroom
+  if (hipMalloc(&c->ws, want) != hipSuccess) {
+    (void)hipGetLastError();
+    c->ws = nullptr;
+    fail(BALE_ERR_ALLOC, "workspace of %zu bytes could not be allocated", want);
+    return nullptr;
+  }
+  c->ws_bytes = want;
+  return c->ws;
 }
 
 static int max_tile_bits_for(const Device* d) {
@@ -185,6 +289,8 @@ int bale_hip_init(int device) {
 
 const char* bale_hip_last_error(void) { return t_err.c_str(); }
 
+int bale_hip_last_strategy(void) { return t_strategy; }
+
 void bale_hip_last_timing(double* gpu_s, double* h2d_s, double* d2h_s) {
   if (gpu_s) *gpu_s = t_gpu_s;
   if (h2d_s) *h2d_s = t_h2d_s;
@@ -225,12 +331,19 @@ int bale_histo(const int64_t* index, int64_t n, void* counts, int64_t nbins, int
   if (dtype != BALE_I64 && dtype != BALE_I32) return fail(BALE_ERR_ARG, "bad dtype %d", dtype);
   if (index_format != BALE_INDEX_PLAIN && index_format != BALE_INDEX_PACKED)
     return fail(BALE_ERR_ARG, "bad index format %d", index_format);
+  if (cfg) {
+    int e = check_config(*cfg, true);
+    if (e) return e;
+  }
   Device* d = device_state();
   if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
   hipStream_t s = (hipStream_t)stream;
+  int e = stream_ready(d, s);
+  if (e) return e;
   bale_config c;
   if (cfg && cfg->strategy != BALE_STRAT_AUTO) c = *cfg;
   else plan_histo(d, n, nbins, dtype, &c);
+  t_strategy = c.strategy;
   bool packed = index_format == BALE_INDEX_PACKED;
   // the bucketed pass keeps 32-bit positions: run it in slices
   const int64_t slice = c.strategy == BALE_STRAT_BUCKET ? ((int64_t)1 << 30) : n;
@@ -244,14 +357,11 @@ int bale_histo(const int64_t* index, int64_t n, void* counts, int64_t nbins, int
       case BALE_STRAT_LDS:
         rc = launch_histo_lds(index + off, m, counts, nbins, dtype, packed, c, d, s);
         break;
-      case BALE_STRAT_BUCKET:
+      default:
         rc = launch_histo_bucket(index + off, m, counts, nbins, dtype, packed, c, d, s);
         break;
-      default:
-        return fail(BALE_ERR_ARG, "bad histogram strategy %d", c.strategy);
     }
-    if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "workspace allocation failed");
-    if (rc) return fail_hip(rc, "histogram launch");
+    if ((rc = launch_result(rc, "histogram launch"))) return rc;
   }
   return BALE_OK;
 }
@@ -264,29 +374,30 @@ int bale_gather(int64_t* tgt, const int64_t* index, int64_t n, const int64_t* ta
   if (table_len <= 0) return fail(BALE_ERR_ARG, "table_len must be positive");
   if (index_format != BALE_INDEX_PLAIN && index_format != BALE_INDEX_PACKED)
     return fail(BALE_ERR_ARG, "bad index format %d", index_format);
+  if (cfg) {
+    int e = check_config(*cfg, false);
+    if (e) return e;
+  }
   Device* d = device_state();
   if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
   hipStream_t s = (hipStream_t)stream;
+  int e = stream_ready(d, s);
+  if (e) return e;
   bale_config c;
   if (cfg && cfg->strategy != BALE_STRAT_AUTO) c = *cfg;
   else plan_gather(d, n, table_len, &c);
+  t_strategy = c.strategy;
   bool packed = index_format == BALE_INDEX_PACKED;
   const int64_t slice = c.strategy == BALE_STRAT_BUCKET ? ((int64_t)1 << 30) : n;
   for (int64_t off = 0; off < n; off += slice) {
     int64_t m = std::min(slice, n - off);
     int rc;
-    switch (c.strategy) {
-      case BALE_STRAT_DIRECT:
-        rc = launch_gather_direct(tgt + off, index + off, m, table, table_len, packed, c, d->err, s);
-        break;
-      case BALE_STRAT_BUCKET:
-        rc = launch_gather_bucket(tgt + off, index + off, m, table, table_len, packed, c, d, s);
-        break;
-      default:
-        return fail(BALE_ERR_ARG, "bad gather strategy %d", c.strategy);
-    }
-    if (rc == hipErrorOutOfMemory) return fail(BALE_ERR_ALLOC, "workspace allocation failed");
-    if (rc) return fail_hip(rc, "gather launch");
+    if (c.strategy == BALE_STRAT_BUCKET)
+      rc = launch_gather_bucket(tgt + off, index + off, m, table, table_len, packed, c, d, s);
+    else
+      rc = launch_gather_direct(tgt + off, index + off, m, table, table_len, packed, c,
+                                stream_err(d, s), s);
+    if ((rc = launch_result(rc, "gather launch"))) return rc;
   }
   return BALE_OK;
 }
@@ -294,13 +405,22 @@ int bale_gather(int64_t* tgt, const int64_t* index, int64_t n, const int64_t* ta
 int bale_sync_errors(void* stream, int64_t* bad) {
   Device* d = device_state();
   if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  int rc = (int)hipStreamSynchronize((hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  int rc = (int)hipStreamSynchronize(s);
   if (rc) return fail_hip(rc, "hipStreamSynchronize");
+  if (bad) *bad = 0;
+  ErrWords* err = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (StreamCtx& c : d->ctx)
+      if (c.used && c.s == s) err = c.err;
+  }
+  if (!err) return BALE_OK;  // no call was made on this stream
   ErrWords h{0, 0}, z{0, 0};
-  rc = (int)hipMemcpy(&h, d->err, sizeof h, hipMemcpyDeviceToHost);
+  rc = (int)hipMemcpy(&h, err, sizeof h, hipMemcpyDeviceToHost);
   if (rc) return fail_hip(rc, "reading the error record");
   if (h.range || h.pe) {
-    rc = (int)hipMemcpy(d->err, &z, sizeof z, hipMemcpyHostToDevice);
+    rc = (int)hipMemcpy(err, &z, sizeof z, hipMemcpyHostToDevice);
     if (rc) return fail_hip(rc, "clearing the error record");
   }
   if (bad) *bad = (int64_t)(h.range + h.pe);
@@ -308,6 +428,22 @@ int bale_sync_errors(void* stream, int64_t* bad) {
                         (unsigned long long)h.pe);
   if (h.range) return fail(BALE_ERR_RANGE, "%llu indices outside the table were skipped",
                            (unsigned long long)h.range);
+  return BALE_OK;
+}
+
+int bale_hip_release_stream(void* stream) {
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (StreamCtx& c : d->ctx) {
+    if (!c.used || c.s != s) continue;
+    int rc = (int)hipStreamSynchronize(s);
+    if (rc) return fail_hip(rc, "hipStreamSynchronize");
+    if (c.ws) (void)hipFree(c.ws);
+    if (c.err) (void)hipFree(c.err);
+    c = StreamCtx();
+  }
   return BALE_OK;
 }
 
@@ -498,13 +634,20 @@ static double dropin(int kind, int64_t* out, int64_t* index, int64_t n, int64_t*
   } else {
     if (kind == 0) plan_histo(d, n, tlen, BALE_I64, &c);
     else plan_gather(d, n, tlen, &c);
-    c.strategy = strategy;
+    const int planned = c.strategy;
+    const int64_t tiles = ceil_div(tlen, (int64_t)1 << max_tile_bits_for(d));
+    int take = strategy;
     // beyond 4096 bucket tiles the bucketed histogram is not offered
-    if (kind == 0 && strategy == BALE_STRAT_BUCKET &&
-        ceil_div(tlen, (int64_t)1 << max_tile_bits_for(d)) > 4096)
-      c.strategy = BALE_STRAT_DIRECT;
-    if (strategy == BALE_STRAT_LDS || strategy == BALE_STRAT_BUCKET) c.block = 1024;
-    if (strategy == BALE_STRAT_LDS && c.unroll == 0) c.unroll = 8;
+    if (kind == 0 && strategy == BALE_STRAT_BUCKET && tiles > 4096) take = BALE_STRAT_DIRECT;
+    // the LDS strategy reads the whole index once per bin tile: beyond one
+    // tile per CU it is slower than the plan by the number of tiles (a 2^26-bin
+    // table would read the index ~2048 times), so the plan runs instead
+    if (kind == 0 && strategy == BALE_STRAT_LDS && tiles > d->cus) take = planned;
+    // a model's own strategy runs without the plan's sample-driven extras
+    if (take != planned) c.hot = 0;
+    c.strategy = take;
+    if (take == BALE_STRAT_LDS || take == BALE_STRAT_BUCKET) c.block = 1024;
+    if (take == BALE_STRAT_LDS && c.unroll == 0) c.unroll = 8;
     c.merge = merge;
   }
   if (kind == 0)

@@ -73,7 +73,7 @@ static int convey_deliver(bale_convey* c, hipStream_t s) {
   PartLayout L = part_carve(ws, C, npe);
   hipLaunchKernelGGL((k_part_count<false, false, false>), dim3((unsigned)C), dim3(1024),
                      (size_t)npe * 4, s, (const int64_t*)c->qpes, n, (uint64_t)npe, 0, npe, chunk,
-                     L.chunk, (const uint32_t*)nullptr, d->err, (const uint32_t*)nullptr);
+                     L.chunk, (const uint32_t*)nullptr, stream_err(d, s), (const uint32_t*)nullptr);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)npe), dim3(256), 0, s, L.chunk, (int)C, npe,
                      L.total);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, npe, 1u << 30, 0u,

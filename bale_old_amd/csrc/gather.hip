@@ -249,7 +249,7 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
     B = ceil_div(table_len, (int64_t)1 << shift);
   }
   int S = part_pick_depth((int)B, sizeof(uint64_t), budget, false);
-  if (S == 0) return (int)hipErrorInvalidValue;
+  if (S == 0) return bad_config("BUCKET gather: %lld table regions do not fit in LDS", (long long)B);
   int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
   int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
   int64_t chunk = ceil_div(n, C);
@@ -266,11 +266,11 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
   size_t lds_count = (size_t)B * 4;
   if (packed)
     hipLaunchKernelGGL((k_part_count<true, false, false>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
-                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err,
+                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, stream_err(d, s),
                        (const uint32_t*)nullptr);
   else
     hipLaunchKernelGGL((k_part_count<false, false, false>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
-                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, d->err,
+                       n, (uint64_t)table_len, shift, (int)B, chunk, L.chunk, nullptr, stream_err(d, s),
                        (const uint32_t*)nullptr);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);

@@ -1039,18 +1039,18 @@ static int direct_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
     grid = std::min<int64_t>(grid, (int64_t)d->cus * gpc);
     dim3 g((unsigned)grid), b(block);
     if (U == 16)
-      hipLaunchKernelGGL((k_histo_direct_hot<16, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, d->err);
+      hipLaunchKernelGGL((k_histo_direct_hot<16, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, stream_err(d, s));
     else if (U == 8)
-      hipLaunchKernelGGL((k_histo_direct_hot<8, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, d->err);
+      hipLaunchKernelGGL((k_histo_direct_hot<8, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, stream_err(d, s));
     else
-      hipLaunchKernelGGL((k_histo_direct_hot<4, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, d->err);
+      hipLaunchKernelGGL((k_histo_direct_hot<4, CT, PACKED>), g, b, 0, s, idx, n, counts, (uint64_t)nbins, hot, stream_err(d, s));
     return (int)hipGetLastError();
   }
-  if (grid > 0x7fffffff) return (int)hipErrorInvalidValue;
+  if (grid > 0x7fffffff) return bad_config("DIRECT histogram: %lld workgroups exceed the grid limit", (long long)grid);
   dim3 g((unsigned)grid), b(block);
-  if (U == 16) go_direct<16, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, d->err);
-  else if (U == 8) go_direct<8, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, d->err);
-  else go_direct<4, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, d->err);
+  if (U == 16) go_direct<16, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, stream_err(d, s));
+  else if (U == 8) go_direct<8, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, stream_err(d, s));
+  else go_direct<4, CT, PACKED>(merge, g, b, s, idx, n, counts, nbins, stream_err(d, s));
   return (int)hipGetLastError();
 }
 
@@ -1098,7 +1098,9 @@ int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins,
   int64_t width = std::min<int64_t>((int64_t)1 << tb, nbins);
   while (tb > 8 && ((int64_t)1 << (tb - 1)) >= width) tb--;  // shrink to the table
   int64_t tiles = ceil_div(nbins, (int64_t)1 << tb);
-  if (tiles > 65535) return (int)hipErrorInvalidValue;
+  if (tiles > 65535)
+    return bad_config("LDS histogram: %lld bin tiles of 2^%d bins exceed 65535; use BUCKET or DIRECT "
+                      "for a table of %lld bins", (long long)tiles, tb, (long long)nbins);
   int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 2;
   int64_t target = (int64_t)d->cus * gpc;
   int64_t chunks = std::max<int64_t>(1, target / tiles);
@@ -1110,16 +1112,26 @@ int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins,
   dim3 g((unsigned)chunks, (unsigned)tiles), b(block);
   size_t lds = (size_t)4 << tb;
   if (dtype == BALE_I64) {
-    if (packed) go_lds_u<int64_t, true>(U, merge, g, b, lds, s, idx, n, (int64_t*)counts, nbins, tb, chunk, d->err);
-    else go_lds_u<int64_t, false>(U, merge, g, b, lds, s, idx, n, (int64_t*)counts, nbins, tb, chunk, d->err);
+    if (packed) go_lds_u<int64_t, true>(U, merge, g, b, lds, s, idx, n, (int64_t*)counts, nbins, tb, chunk, stream_err(d, s));
+    else go_lds_u<int64_t, false>(U, merge, g, b, lds, s, idx, n, (int64_t*)counts, nbins, tb, chunk, stream_err(d, s));
   } else {
-    if (packed) go_lds_u<int32_t, true>(U, merge, g, b, lds, s, idx, n, (int32_t*)counts, nbins, tb, chunk, d->err);
-    else go_lds_u<int32_t, false>(U, merge, g, b, lds, s, idx, n, (int32_t*)counts, nbins, tb, chunk, d->err);
+    if (packed) go_lds_u<int32_t, true>(U, merge, g, b, lds, s, idx, n, (int32_t*)counts, nbins, tb, chunk, stream_err(d, s));
+    else go_lds_u<int32_t, false>(U, merge, g, b, lds, s, idx, n, (int32_t*)counts, nbins, tb, chunk, stream_err(d, s));
   }
   return (int)hipGetLastError();
 }
 
 constexpr int kMaxBuckets = 4096;
+
+static int too_many_buckets(int64_t B, int tb, int64_t nbins) {
+  int need = tb;
+  while (need < 16 && ceil_div(nbins, (int64_t)1 << need) > kMaxBuckets) need++;
+  if (ceil_div(nbins, (int64_t)1 << need) > kMaxBuckets)
+    return bad_config("BUCKET histogram: %lld bins need more than %d buckets of at most 2^16 bins; "
+                      "use DIRECT", (long long)nbins, kMaxBuckets);
+  return bad_config("BUCKET histogram: buckets of 2^%d bins give %lld buckets, more than %d; use "
+                    "tile_log2 >= %d", tb, (long long)B, kMaxBuckets, need);
+}
 
 // Updates per apply segment: long enough that a segment's flush (up to one add
 // per bin of the tile) stays a fraction of its updates, short enough to spread
@@ -1160,13 +1172,15 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
   if (tb > tb_max) tb = tb_max;
   const bool pk = tb > max_tile_bits(d);
   int64_t B = ceil_div(nbins, (int64_t)1 << tb);
-  if (B > kMaxBuckets) return (int)hipErrorInvalidValue;
+  if (B > kMaxBuckets) return too_many_buckets(B, tb, nbins);
   // groups_per_cu scatter workgroups share a CU's LDS, so one waits on
   // memory while another places
   int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
   const size_t budget = (size_t)d->lds_per_block / (size_t)gpc;
   int S = part_pick_depth((int)B, sizeof(uint16_t), budget, HOT);
-  if (S == 0) return (int)hipErrorInvalidValue;
+  if (S == 0)
+    return bad_config("BUCKET histogram: %lld buckets do not fit in the LDS of %d workgroups per CU "
+                      "(use wider buckets or fewer groups_per_cu)", (long long)B, gpc);
   int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * gpc, ceil_div(n, 4096)));
   // chunks are whole multiples of kChunkAlign so both passes see the same
   // aligned 64-update groups
@@ -1182,7 +1196,7 @@ static int bucket_typed(const int64_t* idx, int64_t n, CT* counts, int64_t nbins
     launch_hot_sketch<PACKED>(idx, n, (uint64_t)nbins, L.hot, flags, L.sketch, s);
   size_t lds_count = (size_t)(B + (HOT ? kHotSlots : 0)) * 4;
   hipLaunchKernelGGL((k_part_count<PACKED, HOT, RUNS>), dim3((unsigned)C), dim3(1024), lds_count, s, idx,
-                     n, (uint64_t)nbins, tb, (int)B, chunk, L.chunk, L.hot, d->err,
+                     n, (uint64_t)nbins, tb, (int)B, chunk, L.chunk, L.hot, stream_err(d, s),
                      (const uint32_t*)flags);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk,
                      (int)C, (int)B, L.total);
@@ -1229,7 +1243,7 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   if (tb > tb_max) tb = tb_max;
   const bool pk = tb > max_tile_bits(d);
   int64_t B = ceil_div(nbins, (int64_t)1 << tb);
-  if (B > kMaxBuckets) return (int)hipErrorInvalidValue;
+  if (B > kMaxBuckets) return too_many_buckets(B, tb, nbins);
   int gpc = c.groups_per_cu > 0 ? c.groups_per_cu : 1;
   const size_t budget = (size_t)d->lds_per_block / (size_t)gpc;
   // ring depth with the hot table's LDS taken, and without it (adaptive plan)
@@ -1246,7 +1260,8 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   const int64_t pages = C * part_pool_pages(chunk, (int)B, p_lo);
   const int64_t entries = std::max(C * part_pool_pages(chunk, (int)B, P) * P,
                                    adaptive ? C * part_pool_pages(chunk, (int)B, P_cold) * P_cold : 0);
-  if (entries >= ((int64_t)1 << 32) || pages >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
+  if (entries >= ((int64_t)1 << 32) || pages >= ((int64_t)1 << 31))
+    return bad_config("BUCKET histogram: %lld staged entries exceed 32-bit positions", (long long)entries);
   const size_t head = ((part_words(C, B) * 4) + 255) & ~(size_t)255;
   const size_t tag_b = ((size_t)pages * 4 + 255) & ~(size_t)255;
   const size_t used_b = ((size_t)C * 4 + 255) & ~(size_t)255;
@@ -1266,7 +1281,7 @@ static int bucket_paged_typed(const int64_t* idx, int64_t n, CT* counts, int64_t
   allow_lds(k_histo_scatter_ring<PACKED, HOT, RUNS, CT>, lds);
   hipLaunchKernelGGL((k_histo_scatter_ring<PACKED, HOT, RUNS, CT>), dim3((unsigned)C), dim3(1024),
                      lds, s, idx, n, (uint64_t)nbins, tb, (int)B, R, R_cold, chunk, L.hot, staged,
-                     tags, L.chunk, used, counts, d->err, (const uint32_t*)flags);
+                     tags, L.chunk, used, counts, stream_err(d, s), (const uint32_t*)flags);
   hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk, (int)C, (int)B,
                      L.total);
   // apply segments in pages of the hot-on depth

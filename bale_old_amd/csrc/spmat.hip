@@ -464,14 +464,14 @@ int launch_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const in
   hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, p_offset, (int64_t)1, (int64_t)0);
   if (numrows == 0) return (int)hipGetLastError();
   hipLaunchKernelGGL(k_invert, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, rperminv, numrows,
-                     rperm, d->err);
+                     rperm, stream_err(d, s));
   hipLaunchKernelGGL(k_perm_rowlen, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, offset,
-                     (const int64_t*)rperm, numrows, p_offset + 1, d->err);
+                     (const int64_t*)rperm, numrows, p_offset + 1, stream_err(d, s));
   int rc = scan_incl(p_offset + 1, numrows, stmp, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_perm_fill, dim3((unsigned)ceil_div(numrows, kRowsPerGroup)), dim3(kThreads), 0,
                      s, offset, nonzero, (const int64_t*)rperm, cperminv, numrows, numcols,
-                     (const int64_t*)p_offset, nnz, p_nonzero, d->err);
+                     (const int64_t*)p_offset, nnz, p_nonzero, stream_err(d, s));
   return (int)hipGetLastError();
 }
 

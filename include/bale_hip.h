@@ -234,9 +234,21 @@ int bale_histo(const int64_t *index, int64_t n, void *counts, int64_t nbins,
 int bale_gather(int64_t *tgt, const int64_t *index, int64_t n, const int64_t *table,
                 int64_t table_len, int index_format, const bale_config *cfg, void *stream);
 /* Wait for `stream`, then return BALE_OK, BALE_ERR_RANGE or BALE_ERR_PE for the
- * calls made since the last check on this device, and clear the record.
- * *bad (optional) receives the number of skipped indices. */
+ * calls made on that stream since its last check, and clear its record.
+ * *bad (optional) receives the number of skipped indices.
+ *
+ * Streams: every stream used with this library gets its own error record and
+ * grow-only workspace on its first call (which must not be made while the
+ * stream is being captured into a graph).  Calls on different streams may
+ * overlap; calls on one stream from several host threads must be serialised
+ * by the caller, as the stream serialises their kernels. */
 int bale_sync_errors(void *stream, int64_t *bad);
+/* Wait for `stream` and free its error record and workspace (for callers that
+ * create and destroy many streams; at most 256 are held per device). */
+int bale_hip_release_stream(void *stream);
+/* The strategy (BALE_STRAT_DIRECT, _LDS or _BUCKET) the last bale_histo,
+ * bale_gather or drop-in call on this thread ran; BALE_STRAT_AUTO before any. */
+int bale_hip_last_strategy(void);
 /* The configuration BALE_STRAT_AUTO would run for this shape. */
 int bale_histo_plan(int64_t n, int64_t nbins, int dtype, bale_config *out);
 int bale_gather_plan(int64_t n, int64_t table_len, bale_config *out);
@@ -318,7 +330,12 @@ typedef struct bale_instance {
 typedef struct bale_batch bale_batch;
 /* Plan `count` instances into per-phase task lists and capture the launch
  * sequence in one HIP graph.  Instances keep their pointers; relaunching
- * repeats the work (histograms add again). NULL on error. */
+ * repeats the work (histograms add again). NULL on error.
+ * Histogram instances may share counts (several index streams into one
+ * table): instances whose counts overlap are flushed with atomics only.  A
+ * gather's table or tgt must not overlap any histogram's counts in the same
+ * batch (the phases do not order them).  Index errors of a launch are
+ * reported by bale_sync_errors on the launching stream. */
 bale_batch *bale_batch_create(const bale_instance *inst, int64_t count, void *stream);
 /* One graph launch of the whole batch on `stream`. */
 int bale_batch_launch(bale_batch *b, void *stream);

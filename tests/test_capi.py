@@ -85,6 +85,34 @@ def test_host_side_argument_checks():
     assert L.bale_probe_copy(1, 1, 15, None) == B.ERR_ARG
 
 
+@pytest.mark.parametrize("field,value,words", [
+    ("strategy", 7, "strategy"), ("block", 300, "block"), ("unroll", 5, "unroll"),
+    ("tile_log2", 40, "tile_log2"), ("merge", 3, "merge"), ("groups_per_cu", 99, "groups_per_cu"),
+    ("hot", 3, "hot"), ("task_log2", 5, "task_log2"),
+])
+def test_configs_are_checked_on_the_host(field, value, words):
+    """A configuration no shape can take is refused with BALE_ERR_ARG and a
+    reason before any device is touched (the pointers are never read)."""
+    L = B.lib()
+    cfg = B.config(B.BUCKET)
+    setattr(cfg, field, value)
+    assert L.bale_histo(1, 5, 1, 10, B.I64, B.PLAIN, C.byref(cfg), None) == B.ERR_ARG
+    assert words in B.last_error()
+    cfg = B.config(B.DIRECT)
+    setattr(cfg, field, value)
+    if field != "hot" or value != 2:
+        assert L.bale_gather(1, 1, 5, 1, 10, B.PLAIN, C.byref(cfg), None) == B.ERR_ARG
+
+
+def test_hot2_and_lds_are_histogram_or_bucket_only():
+    L = B.lib()
+    cfg = B.config(B.DIRECT, hot=2)
+    assert L.bale_histo(1, 5, 1, 10, B.I64, B.PLAIN, C.byref(cfg), None) == B.ERR_ARG
+    cfg = B.config(B.LDS)
+    assert L.bale_gather(1, 1, 5, 1, 10, B.PLAIN, C.byref(cfg), None) == B.ERR_ARG
+    assert "gather strategy" in B.last_error()
+
+
 def test_adjacent_kernels_check_arguments_on_the_host():
     """randperm, transpose, permute, generator, toposort: sizes and modes are
     refused before any device is touched (the pointers are never read)."""
