@@ -95,6 +95,9 @@ struct Device {
   int id = -1;
   int cus = 0;
   int lds_per_block = 0;
+  // this device's address of the no-allocation scratch array (capi.hip),
+  // resolved once when the device's state is created; nullptr if that failed
+  void* scratch = nullptr;
   StreamCtx ctx[kMaxStreamCtx];
 };
 Device* device_state();  // current device's state (created on first use; allocates nothing)

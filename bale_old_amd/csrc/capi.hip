@@ -66,6 +66,27 @@ constexpr int kMaxDevices = 64;
 static Device g_dev[kMaxDevices];
 static std::mutex g_mu;
 
+// ---- the no-allocation entry point's scratch ------------------------------
+// One fixed array per device (each GPU the library runs on holds its own copy
+// until the process ends).  In host code `scratch` names a host-side stand-in,
+// not a device address: host code uses only Device::scratch, the address
+// hipGetSymbolAddress gives for the current device.  Layout: 256 bytes of
+// error words for calls that pass no error counters, then the workspace.
+constexpr size_t kScratchBytes = (size_t)BALE_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kScratchErrBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// While a no-allocation call runs on this thread, workspace() and stream_err()
+// hand out the scratch array instead of the stream's state (which may need
+// allocating); `wanted` records the largest workspace asked for.
+struct NoAllocScope {
+  void* ws;
+  size_t ws_bytes;
+  ErrWords* err;
+  size_t wanted = 0;
+};
+static thread_local NoAllocScope* t_noalloc = nullptr;
+
 Device* device_state() {
   int id = 0;
   if (hipGetDevice(&id) != hipSuccess || id < 0 || id >= kMaxDevices) return nullptr;
@@ -83,6 +104,14 @@ Device* device_state() {
     lds = optin;
   (void)hipGetLastError();
   d->lds_per_block = lds;
+  // the scratch array's address on this device (the current one)
+  void* sp = nullptr;
+  if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) == hipSuccess && sp) {
+    d->scratch = sp;
+  } else {
+    (void)hipGetLastError();
+    d->scratch = nullptr;
+  }
   d->id = id;
   return d;
 }
@@ -134,6 +163,7 @@ StreamCtx* stream_ctx(Device* d, hipStream_t s) {
 }
 
 ErrWords* stream_err(Device* d, hipStream_t s) {
+  if (t_noalloc) return t_noalloc->err;
   StreamCtx* c = stream_ctx(d, s);
   return c ? c->err : nullptr;
 }
@@ -173,6 +203,10 @@ static int check_config(const bale_config& c, bool histogram) {
 }
 
 void* workspace(Device* d, size_t bytes, hipStream_t s) {
+  if (t_noalloc) {
+    t_noalloc->wanted = std::max(t_noalloc->wanted, bytes);
+    return bytes <= t_noalloc->ws_bytes ? t_noalloc->ws : nullptr;
+  }
   StreamCtx* c = stream_ctx(d, s);
   if (!c) return nullptr;
   if (c->ws_bytes >= bytes) return c->ws;
@@ -322,8 +356,9 @@ int bale_gather_plan(int64_t n, int64_t table_len, bale_config* out) {
   return BALE_OK;
 }
 
-int bale_histo(const int64_t* index, int64_t n, void* counts, int64_t nbins, int dtype,
-               int index_format, const bale_config* cfg, void* stream) {
+// Arguments of bale_histo / bale_histo_noalloc that no device is needed to refuse.
+static int check_histo_args(const int64_t* index, int64_t n, void* counts, int64_t nbins,
+                            int dtype, int index_format, const bale_config* cfg) {
   if (n < 0) return fail(BALE_ERR_ARG, "negative update count %lld", (long long)n);
   if (n == 0) return BALE_OK;
   if (!index || !counts) return fail(BALE_ERR_ARG, "null index or counts");
@@ -331,15 +366,13 @@ int bale_histo(const int64_t* index, int64_t n, void* counts, int64_t nbins, int
   if (dtype != BALE_I64 && dtype != BALE_I32) return fail(BALE_ERR_ARG, "bad dtype %d", dtype);
   if (index_format != BALE_INDEX_PLAIN && index_format != BALE_INDEX_PACKED)
     return fail(BALE_ERR_ARG, "bad index format %d", index_format);
-  if (cfg) {
-    int e = check_config(*cfg, true);
-    if (e) return e;
-  }
-  Device* d = device_state();
-  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
-  hipStream_t s = (hipStream_t)stream;
-  int e = stream_ready(d, s);
-  if (e) return e;
+  if (cfg) return check_config(*cfg, true);
+  return BALE_OK;
+}
+
+// The histogram's launches (shared by bale_histo and bale_histo_noalloc).
+static int histo_run(Device* d, hipStream_t s, const int64_t* index, int64_t n, void* counts,
+                     int64_t nbins, int dtype, int index_format, const bale_config* cfg) {
   bale_config c;
   if (cfg && cfg->strategy != BALE_STRAT_AUTO) c = *cfg;
   else plan_histo(d, n, nbins, dtype, &c);
@@ -361,9 +394,39 @@ int bale_histo(const int64_t* index, int64_t n, void* counts, int64_t nbins, int
         rc = launch_histo_bucket(index + off, m, counts, nbins, dtype, packed, c, d, s);
         break;
     }
+    if (rc == hipErrorOutOfMemory && t_noalloc)
+      return fail(BALE_ERR_ARG, "this histogram needs %zu bytes of scratch, more than the %zu the "
+                                "no-allocation entry point has (use bale_histo)",
+                  t_noalloc->wanted, t_noalloc->ws_bytes);
     if ((rc = launch_result(rc, "histogram launch"))) return rc;
   }
   return BALE_OK;
+}
+
+int bale_histo(const int64_t* index, int64_t n, void* counts, int64_t nbins, int dtype,
+               int index_format, const bale_config* cfg, void* stream) {
+  int e = check_histo_args(index, n, counts, nbins, dtype, index_format, cfg);
+  if (e || n == 0) return e;
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  hipStream_t s = (hipStream_t)stream;
+  if ((e = stream_ready(d, s))) return e;
+  return histo_run(d, s, index, n, counts, nbins, dtype, index_format, cfg);
+}
+
+int bale_histo_noalloc(const int64_t* index, int64_t n, void* counts, int64_t nbins, int dtype,
+                       int index_format, const bale_config* cfg, int64_t* errors, void* stream) {
+  int e = check_histo_args(index, n, counts, nbins, dtype, index_format, cfg);
+  if (e || n == 0) return e;
+  Device* d = device_state();
+  if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
+  if (!d->scratch) return fail(BALE_ERR_HIP, "the scratch array's device address is unknown");
+  NoAllocScope scope{(char*)d->scratch + kScratchErrBytes, kScratchBytes - kScratchErrBytes,
+                     errors ? (ErrWords*)errors : (ErrWords*)d->scratch};
+  t_noalloc = &scope;
+  e = histo_run(d, (hipStream_t)stream, index, n, counts, nbins, dtype, index_format, cfg);
+  t_noalloc = nullptr;
+  return e;
 }
 
 int bale_gather(int64_t* tgt, const int64_t* index, int64_t n, const int64_t* table,

@@ -230,6 +230,23 @@ void bale_hip_last_timing(double *gpu_s, double *h2d_s, double *d2h_s);
  * bale_sync_errors(). */
 int bale_histo(const int64_t *index, int64_t n, void *counts, int64_t nbins,
                int dtype, int index_format, const bale_config *cfg, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): bale_histo with the same results, whose workspace is one
+ * fixed array of BALE_NOALLOC_SCRATCH_BYTES per device instead of memory the
+ * library allocates; it calls no hipMalloc or hipFree.  A call whose plan
+ * needs more scratch than that returns BALE_ERR_ARG, saying how much (about
+ * 2 bytes per update for the bucketed plan, so up to ~2^24 updates; DIRECT
+ * and LDS need none beyond the hot-bin sample, 40 KiB).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (on other streams or
+ * threads) race, and the caller must order them.  errors: NULL, or device
+ * memory for two int64 counters {indices outside the table, packed indices
+ * naming another PE} that the call adds to (they are not reported through
+ * bale_sync_errors). */
+#define BALE_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int bale_histo_noalloc(const int64_t *index, int64_t n, void *counts, int64_t nbins,
+                       int dtype, int index_format, const bale_config *cfg, int64_t *errors,
+                       void *stream);
 /* tgt[i] = table[index[i]] for i < n. */
 int bale_gather(int64_t *tgt, const int64_t *index, int64_t n, const int64_t *table,
                 int64_t table_len, int index_format, const bale_config *cfg, void *stream);
