@@ -276,6 +276,7 @@ __device__ inline void pk_tally_add(const PkTally& t, uint32_t bin, uint32_t amo
 template <typename CT>
 __device__ inline void pk_account(uint32_t bin, uint32_t inc, uint32_t old, CT* dst,
                                   const PkTally& t) {
+  if (__builtin_expect(!pk_wrapped(old, bin & 1u, inc), 1)) return;  // one compare per add
   const PkOwed o = pk_owed(old, bin & 1u, inc);
   if (o.own) pk_tally_add(t, bin, o.own, dst);
   if (o.high && (bin | 1u) < t.width) pk_tally_add(t, bin | 1u, o.high, dst);

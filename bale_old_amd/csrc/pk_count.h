@@ -34,6 +34,12 @@ struct PkOwed {
   uint32_t high;  // owed to the high bin of the word (low-half adds only)
 };
 
+// Whether an add of inc to `half` wrapped that half: the only case in which
+// pk_owed owes anything (a low half's carry needs its own wrap).
+__host__ __device__ inline bool pk_wrapped(uint32_t old, uint32_t half, uint32_t inc) {
+  return ((old >> (16u * half)) & 0xffffu) + inc >= 0x10000u;
+}
+
 // old: the word's value the atomic add returned; half: 0 for the low bin (even
 // bin number), 1 for the high bin; inc: what was added to that half (< 2^16).
 __host__ __device__ inline PkOwed pk_owed(uint32_t old, uint32_t half, uint32_t inc) {

@@ -37,6 +37,8 @@ extern "C" int pk_replay(const uint32_t* bin, const uint32_t* inc, const int64_t
       const int64_t a = order[next++];
       if (a > k) return -2;  // accounted before its own atomic
       const bale::PkOwed o = bale::pk_owed(returned[a], bin[a] & 1u, inc[a]);
+      // the kernels skip pk_owed unless pk_wrapped: it must owe nothing then
+      if (!bale::pk_wrapped(returned[a], bin[a] & 1u, inc[a]) && (o.own || o.high)) return -4;
       tally[bin[a]] += o.own;
       if (o.high && (bin[a] | 1u) < width) tally[bin[a] | 1u] += o.high;
     }
