@@ -197,8 +197,8 @@ static int check_config(const bale_config& c, bool histogram) {
     return fail(BALE_ERR_ARG, "groups_per_cu %d: 0 (default) or 1..16", c.groups_per_cu);
   if (c.hot < 0 || c.hot > 2 || (c.hot == 2 && c.strategy != BALE_STRAT_BUCKET && c.strategy != BALE_STRAT_AUTO))
     return fail(BALE_ERR_ARG, "hot %d: 0, 1, or 2 with the BUCKET strategy", c.hot);
-  if (c.task_log2 != 0 && (c.task_log2 < 12 || c.task_log2 > 30))
-    return fail(BALE_ERR_ARG, "task_log2 %d: 0 (default) or 12..30", c.task_log2);
+  if (c.task_log2 != 0 && (c.task_log2 < 8 || c.task_log2 > 30))
+    return fail(BALE_ERR_ARG, "task_log2 %d: 0 (default) or 8..30", c.task_log2);
   return BALE_OK;
 }
 
@@ -269,7 +269,7 @@ static void plan_histo(const Device* d, int64_t n, int64_t nbins, int dtype, bal
     c->unroll = 8;
   }
 }
-static void plan_gather(const Device*, int64_t, int64_t, bale_config* c) {
+static void plan_gather(const Device*, int64_t n, int64_t table_len, bale_config* c) {
   memset(c, 0, sizeof *c);
   c->strategy = BALE_STRAT_DIRECT;
   c->block = 256;
@@ -277,6 +277,22 @@ static void plan_gather(const Device*, int64_t, int64_t, bale_config* c) {
   // table on MI355X, against 6.79 ms with 16 and 6.73 ms with 4
   c->unroll = 8;
   c->merge = BALE_MERGE_NONE;
+  // Large uniform gathers go by table region (BUCKET); a sample taken on the
+  // device sends inputs that repeat entries (hot entries, clustered runs) to
+  // DIRECT, which the caches serve better.  Measured on MI355X, 2^28 uniform
+  // requests: 56.6-58.9 against 40.9 GUPS into a 2^28-entry table, 67.6 against
+  // 54.0 into 2^22 entries; Zipf(1.1) into 2^28 entries: DIRECT 123 GUPS,
+  // BUCKET 68.  BUCKET reads every region's table lines once however few
+  // requests there are: even at 2^26 requests into 2^28 entries, and behind
+  // DIRECT at 2^24, hence at least one request per two entries.
+  if (n >= ((int64_t)1 << 24) && 2 * n >= table_len && table_len >= ((int64_t)1 << 22) &&
+      table_len <= ((int64_t)1 << 30)) {
+    c->strategy = BALE_STRAT_BUCKET;
+    c->hot = 2;
+    c->tile_log2 = 18;
+    c->task_log2 = 11;
+    c->groups_per_cu = 4;
+  }
 }
 
 bool is_device_ptr(const void* p) {

@@ -69,14 +69,17 @@ def median(xs):
     return s[len(s) // 2]
 
 
-def bench_gather(B, torch, dev, steps, warmup, stream, n_log2=28, t_log2=28):
+def bench_gather(B, torch, dev, steps, warmup, stream, n_log2=28, t_log2=28, tensors=None, cfg=None):
     n, tlen = 1 << n_log2, 1 << t_log2
-    table = torch.empty(tlen, dtype=torch.int64, device=dev)
-    idx = torch.empty(n, dtype=torch.int64, device=dev)
-    tgt = torch.empty(n, dtype=torch.int64, device=dev)
-    B.fill_splitmix(table, stream)
-    B.gen_index(idx, tlen, dist=0, seed=2, stream=stream)
-    cfg = B.gather_plan(n, tlen)
+    if tensors is None:
+        table = torch.empty(tlen, dtype=torch.int64, device=dev)
+        idx = torch.empty(n, dtype=torch.int64, device=dev)
+        tgt = torch.empty(n, dtype=torch.int64, device=dev)
+        B.fill_splitmix(table, stream)
+        B.gen_index(idx, tlen, dist=0, seed=2, stream=stream)
+    else:
+        table, idx, tgt = tensors
+    cfg = cfg or B.gather_plan(n, tlen)
 
     def step():
         B.gather(tgt, idx, table, cfg=cfg, stream=stream, sync=False)
@@ -232,52 +235,72 @@ def probes(B, torch, dev, stream, table):
             "random_read_table_bytes": table.numel() * 8}
 
 
-def cpu_baseline(idx_t, table_t, max_threads=16):
-    """The oracle's multi-threaded AGI gather (relaxed loads) on the host cores,
-    timed like the reference (per-thread wall seconds between barriers,
-    averaged over threads, libgetput/libgetput.upc:395-401)."""
+def cpu_share():
+    """Threads for the CPU baselines: the CPU share this process may use (the
+    cores it is bound to, at most OMP_NUM_THREADS when that is set, as on the
+    GPU boxes), with nproc and the affinity recorded beside it."""
+    aff = len(os.sched_getaffinity(0))
+    share = aff
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        share = min(aff, int(env))
+    return share, {"nproc": os.cpu_count(), "affinity_cpus": aff, "threads_used": share,
+                   "limit": "OMP_NUM_THREADS" if share < aff else "affinity"}
+
+
+def cpu_baseline(idx_t, table_t):
+    """The reference's two models of the gather on the host cores, timed like
+    the reference (per-PE wall seconds between barriers, averaged over PEs,
+    libgetput/libgetput.upc:395-401): AGI (relaxed loads, apps/ig_src/ig_agi.upc:61-64)
+    and aggregate-by-owner request/reply (apps/ig_src/ig_exstack.upc:69-94),
+    at P = the CPU share and P = 1."""
     import numpy as np
     import oracle
-    threads = max(1, min(max_threads, len(os.sched_getaffinity(0))))
+    P, info = cpu_share()
     n = idx_t.numel()
     sample = min(n, 1 << 26)
     idx = idx_t[:sample].cpu().numpy()
     table = table_t.cpu().numpy()
     tgt = np.empty(sample, np.int64)
-    oracle.ig_agi_mt(idx, table, tgt, threads)  # warm-up
-    secs = []
-    for _ in range(3):
-        secs.append(oracle.ig_agi_mt(idx, table, tgt, threads))
-    t = sum(secs) / len(secs)
-    # one thread (P = 1) on a smaller sample of the same requests
     s1 = min(n, 1 << 23)
-    oracle.ig_agi_mt(idx[:s1], table, tgt[:s1], 1)
-    t1 = sum(oracle.ig_agi_mt(idx[:s1], table, tgt[:s1], 1) for _ in range(3)) / 3
-    return {"value": sample / t / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+
+    def timed(fn, m, threads):
+        fn(idx[:m], table, tgt[:m], threads)  # warm-up
+        t = sum(fn(idx[:m], table, tgt[:m], threads) for _ in range(3)) / 3
+        assert np.array_equal(tgt[:m], table[idx[:m]])
+        return m / t / 1e9
+
+    models = {}
+    for name, fn in (("agi", oracle.ig_agi_mt), ("owner", oracle.ig_owner_mt)):
+        models[name] = {f"P{P}": timed(fn, sample, P), "P1": timed(fn, s1, 1)}
+    best = max(models, key=lambda k: models[k][f"P{P}"])
+    return {"value": models[best][f"P{P}"], "unit": "GUPS", "cores": P, "kind": "port",
+            "model": best, "models_GUPS": models, "host": info,
             "sample": f"first 2^{sample.bit_length() - 1} of the 2^28 requests into the full 2^28-entry "
-                      f"table (2 GiB), oracle or_ig_agi_mt with {threads} threads, mean of 3 after 1 warm-up",
-            "one_thread": {"value": s1 / t1 / 1e9, "unit": "GUPS", "cores": 1,
-                           "sample": f"first 2^{s1.bit_length() - 1} requests, same table"}}
+                      f"table (2 GiB) with {P} threads (2^{s1.bit_length() - 1} with 1), the oracle's "
+                      f"AGI and owner models, mean of 3 after 1 warm-up; value = the faster at P={P}"}
 
 
-def cpu_histo(max_threads=16):
-    """The oracle's AGI histogram (relaxed atomic adds on shared counters,
-    apps/histo_src/histo_agi.upc:61-66) on the host cores and on one core:
+def cpu_histo():
+    """The reference's AGI histogram (relaxed atomic adds on shared counters,
+    apps/histo_src/histo_agi.upc:61-66) and aggregate-by-owner one
+    (apps/histo_src/histo_exstack.upc:64-76) on the host cores and on one core:
     uniform updates into 2^26 int64 counters, the bench's 2^28-update case
     sampled to 2^26 (2^24 for one thread)."""
     import numpy as np
     import oracle
-    threads = max(1, min(max_threads, len(os.sched_getaffinity(0))))
+    P, info = cpu_share()
     nbins = 1 << 26
     idx = oracle.gen_index(1 << 26, nbins, 0, 1.0, 0, 3)
     counts = np.zeros(nbins, np.int64)
-    res = {}
-    for cores, n in ((threads, idx.size), (1, 1 << 24)):
-        oracle.histo_agi_mt(idx[:n], counts, cores)
-        t = sum(oracle.histo_agi_mt(idx[:n], counts, cores) for _ in range(3)) / 3
-        res[f"{cores}_threads"] = {"value": n / t / 1e9, "unit": "GUPS", "cores": cores, "kind": "port",
-                                   "sample": f"2^{n.bit_length() - 1} uniform updates (seed 3) into 2^26 "
-                                             f"int64 counters, mean of 3 after 1 warm-up"}
+    res = {"host": info}
+    for name, fn in (("agi", oracle.histo_agi_mt), ("owner", oracle.histo_owner_mt)):
+        for cores, n in ((P, idx.size), (1, 1 << 24)):
+            fn(idx[:n], counts, cores)
+            t = sum(fn(idx[:n], counts, cores) for _ in range(3)) / 3
+            res[f"{name}_P{cores}"] = {"value": n / t / 1e9, "unit": "GUPS", "cores": cores, "kind": "port",
+                                       "sample": f"2^{n.bit_length() - 1} uniform updates (seed 3) into 2^26 "
+                                                 f"int64 counters, mean of 3 after 1 warm-up"}
     return res
 
 
@@ -456,6 +479,68 @@ def cpu_spmat():
     return res
 
 
+WORKLOADS = {
+    "gather": "the planned gather (by table region) of 2^28 uniform requests into a 2^28-entry table",
+    "gather_direct": "k_gather_direct over the same requests",
+    "copy": "k_probe_copy of exactly 1 GiB (read 1 GiB, write 1 GiB): the counters' calibration",
+    "histo": "the planned histogram of 2^28 uniform updates into 2^26 int64 bins",
+    "zipf": "the same plan on 2^28 Zipf(1.1) updates (its sample switches the hot table on)",
+    "small": "the planned histogram of 2^24 uniform updates into 2^20 int64 bins",
+    "batch": "the held-out mix of 4096 instances (seed 5), learned choices, launched three times",
+}
+
+
+def run_workloads(which) -> int:
+    """One call of each named workload and nothing else, for hardware-counter
+    runs (rocprofv3 --pmc ... -- python3 bench.py --workload gather copy)."""
+    import torch
+    import bale_old_amd as B
+    dev = torch.device("cuda:0")
+    B.check(B.lib().bale_hip_init(0))
+    st = torch.cuda.current_stream()
+    n = 1 << 28
+    idx = torch.empty(n, dtype=torch.int64, device=dev)
+    if which & {"gather", "gather_direct", "copy"}:
+        table = torch.empty(n, dtype=torch.int64, device=dev)
+        B.fill_splitmix(table)
+        tgt = torch.empty(n, dtype=torch.int64, device=dev)
+        B.gen_index(idx, n, seed=2)
+        if "gather" in which:
+            B.gather(tgt, idx, table, cfg=B.gather_plan(n, n))
+        if "gather_direct" in which:
+            B.gather(tgt, idx, table, cfg=B.config(B.DIRECT, block=256, unroll=8))
+        if "copy" in which:
+            B.check(B.lib().bale_probe_copy(tgt.data_ptr(), table.data_ptr(), 1 << 30, st.cuda_stream))
+        torch.cuda.synchronize()
+        del table, tgt
+    for name, nb, dist in (("histo", 1 << 26, 0), ("zipf", 1 << 26, 1)):
+        if name in which:
+            counts = torch.zeros(nb, dtype=torch.int64, device=dev)
+            B.gen_index(idx, nb, dist=dist, s=1.1, seed=3)
+            B.histo(idx, counts, cfg=B.histo_plan(n, nb))
+            del counts
+    if "small" in which:
+        m, nb = 1 << 24, 1 << 20
+        counts = torch.zeros(nb, dtype=torch.int64, device=dev)
+        B.gen_index(idx[:m], nb, seed=1)
+        B.histo(idx[:m], counts, cfg=B.histo_plan(m, nb))
+        del counts
+    if "batch" in which:
+        del idx
+        from bale_old_amd import mix as MX, selector as S
+        mix = MX.Mix(MX.mix_params(4096, 5), dev, st)
+        weights, _ = S.load(dev)
+        S.select(mix.inst, 4096, weights, dev, st)
+        b = MX.Batch(mix.inst, 4096, st)
+        for _ in range(3):
+            b.launch(st)
+        torch.cuda.synchronize()
+        b.close()
+    torch.cuda.synchronize()
+    log("workloads done: " + " ".join(sorted(which)))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,7 +550,12 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="headline gather only")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixes")
     ap.add_argument("--batch-instances", type=int, default=4096)
+    ap.add_argument("--workload", nargs="+", choices=sorted(WORKLOADS),
+                    help="run only these workloads once each (for rocprofv3 counter runs) and exit: "
+                         + "; ".join(f"{k}: {v}" for k, v in WORKLOADS.items()))
     args = ap.parse_args()
+    if args.workload:
+        return run_workloads(set(args.workload))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the exchange between GPUs is designed, "
               "not built or measured; run with --gpus 1", file=sys.stderr)
@@ -482,7 +572,7 @@ def main() -> int:
     cus, lds = B.device_info()
     log(f"device: {torch.cuda.get_device_name(0)} cus={cus} lds/wg={lds}")
 
-    log("gather 2^28 x 2^28 ...")
+    log("gather 2^28 x 2^28 (planned: by table region, sample-checked) ...")
     g = bench_gather(B, torch, dev, args.steps, args.warmup, stream)
     table, idx, tgt = g.pop("tensors")
     ms = g["total_s"] / args.steps * 1e3
@@ -490,9 +580,17 @@ def main() -> int:
     kern_s = sum(g["per_step_s"]) / len(g["per_step_s"])
     alg_bytes = 24 * g["n"]  # 8 B index + 8 B table + 8 B tgt per request
     achieved = alg_bytes / kern_s / 1e9
-    log(f"gather: {value:.2f} GUPS, {ms:.3f} ms/step, {achieved:.0f} GB/s algorithmic")
+    strategy = {B.DIRECT: "direct", B.BUCKET: "bucket"}.get(g["cfg"]["strategy"], str(g["cfg"]["strategy"]))
+    log(f"gather: {value:.2f} GUPS, {ms:.3f} ms/step, {achieved:.0f} GB/s algorithmic ({strategy})")
 
     extra = {"gather_cfg": g["cfg"], "gather_median_step_ms": median(g["per_step_s"]) * 1e3}
+    # the one-load-per-request kernel on the same tensors, for comparison
+    gd = bench_gather(B, torch, dev, args.steps, args.warmup, stream, tensors=(table, idx, tgt),
+                      cfg=B.config(B.DIRECT, block=256, unroll=8))
+    gd.pop("tensors")
+    extra["gather_direct"] = {"GUPS": gd["n"] * args.steps / gd["total_s"] / 1e9,
+                              "ms_per_step": gd["total_s"] / args.steps * 1e3, "cfg": gd["cfg"]}
+    log(f"gather direct: {extra['gather_direct']}")
     if not args.no_extra:
         log("probes ...")
         pr = probes(B, torch, dev, stream, table)
@@ -576,10 +674,14 @@ def main() -> int:
         "data": "synthetic (seeded splitmix64 streams generated on the GPU)",
         "config": {"workload": "index gather: 2^28 int64 requests, uniform (splitmix64 seed 2) over a "
                                "2^28-entry int64 table with table[j]=splitmix64(j); inputs resident in HBM",
-                   "requests": g["n"], "table_len": g["table_len"], "strategy": "direct"},
+                   "requests": g["n"], "table_len": g["table_len"], "strategy": strategy},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("gather_direct_2p28"),
-                     "kernel": "k_gather_direct", "kernel_avg_ms": kern_s * 1e3,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic("gather_bucket_2p28" if strategy == "bucket" else "gather_direct_2p28"),
+                     "kernel": ("bucketed gather call: k_gsort + k_gtranspose + k_gregion + k_gunpermute "
+                                "(+ the sample and the DIRECT launch it skips)" if strategy == "bucket"
+                                else "k_gather_direct"),
+                     "kernel_avg_ms": kern_s * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "extra": extra,

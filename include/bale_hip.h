@@ -209,7 +209,8 @@ typedef struct bale_config {
                          BUCKET only: 2 = let that sample decide on the device, per
                          call, whether to use the hot table and run merging, and (with
                          task_log2 = 0) the apply segment length */
-  int32_t task_log2;  /* batches: operations per task (log2, 12..22); 0 = 16.
+  int32_t task_log2;  /* 0 or 8..30.  Batches: operations per task (log2, 12..22); 0 = 16.
+                         BUCKET gathers: about 2^task_log2 requests per region segment.
                          BUCKET histograms: updates per apply segment (log2); 0 = by size
                          (with hot = 2: twice the average bucket when the sample finds
                          no hot bins, half of it, at least 16384, when it does) */

@@ -51,6 +51,8 @@ def lib():
             "or_histo_agi_i32": (None, [P, I, P]),
             "or_histo_owner": (C.c_int, [P, I, P, I, I]),
             "or_histo_agi_mt": (C.c_double, [P, I, P, C.c_int]),
+            "or_histo_owner_mt": (C.c_double, [P, I, P, C.c_int]),
+            "or_ig_owner_mt": (C.c_double, [P, P, I, P, C.c_int]),
             "or_ig_agi": (None, [P, P, I, P]),
             "or_ig_exstack": (C.c_int, [P, P, I, P, I, I]),
             "or_ig_conveyor": (C.c_int, [P, P, I, P, I, I]),
@@ -172,6 +174,16 @@ def ig_conveyor(pck: np.ndarray, ltables: np.ndarray, P: int) -> np.ndarray:
     rc = lib().or_ig_conveyor(_p(tgt), _p(pck), pck.size, _p(ltables), ltables.size // P, P)
     assert rc == 0
     return tgt
+
+
+def histo_owner_mt(index: np.ndarray, counts: np.ndarray, nthreads: int) -> float:
+    """Aggregate-by-owner histogram over nthreads PEs; average seconds per PE."""
+    return lib().or_histo_owner_mt(_p(index), index.size, _p(counts), nthreads)
+
+
+def ig_owner_mt(index: np.ndarray, table: np.ndarray, tgt: np.ndarray, nthreads: int) -> float:
+    """Request/reply gather by owner over nthreads PEs; average seconds per PE."""
+    return lib().or_ig_owner_mt(_p(tgt), _p(index), index.size, _p(table), nthreads)
 
 
 def ig_agi_mt(index: np.ndarray, table: np.ndarray, tgt: np.ndarray, nthreads: int) -> float:

@@ -345,6 +345,134 @@ double or_ig_agi_mt(int64_t *tgt, const int64_t *index, int64_t n,
   return run_mt(index, n, NULL, tgt, table, nthreads, 1);
 }
 
+/* ---- timed multi-thread aggregate-by-owner baseline -------------------- */
+/* P threads play P PEs; global index j belongs to PE j % P (the cyclic layout,
+ * libgetput/libgetput.h:159,186-187).  Bulk-synchronous rounds as exstack runs
+ * them (exstack/exstack.upc, apps/histo_src/histo_exstack.upc:64-76,
+ * apps/ig_src/ig_exstack.upc:69-94): each PE pushes its items into a buffer of
+ * at most kOwnBuf per destination until one fills or its input ends; after a
+ * barrier every owner applies (histogram) or answers in place (gather) what was
+ * sent to it; the gather adds a second barrier after which each requester
+ * retraces its round's requests in order, popping each source's replies; a
+ * closing barrier frees the buffers.  Rounds repeat until every PE's input is
+ * done.  The shared arrays are the global counts / table / tgt: each owner
+ * touches only its own entries, so no atomics are needed. */
+enum { kOwnBuf = 1024 };
+typedef struct own_arg {
+  const int64_t *index;
+  int64_t lo, hi;
+  int64_t *counts;
+  int64_t *tgt;
+  const int64_t *table;
+  int64_t *buf;    /* [P][P][kOwnBuf]: buf[src][dst] */
+  int64_t *cnt;    /* [P][P] items in buf[src][dst] */
+  int64_t *done;   /* [P] round in which the PE's input ended (or INT64_MAX) */
+  pthread_barrier_t *bar;
+  int me, P, gather;
+  double secs;
+} own_arg;
+
+static void *own_body(void *p) {
+  own_arg *a = (own_arg *)p;
+  const int P = a->P, me = a->me;
+  int64_t *mycnt = a->cnt + (int64_t)me * P;
+  int64_t *pos = malloc(sizeof(int64_t) * (size_t)P);
+  pthread_barrier_wait(a->bar);
+  double t0 = wall_seconds();
+  int64_t i = a->lo;
+  for (int64_t round = 0;; round++) {
+    /* push: until a destination buffer fills or the input ends */
+    for (int d = 0; d < P; d++) mycnt[d] = 0;
+    const int64_t i0 = i;
+    for (; i < a->hi; i++) {
+      const int64_t j = a->index[i];
+      const int d = (int)(j % P);
+      if (mycnt[d] == kOwnBuf) break;
+      a->buf[((int64_t)me * P + d) * kOwnBuf + mycnt[d]++] = j / P;
+    }
+    if (i == a->hi && a->done[me] == INT64_MAX) a->done[me] = round;
+    pthread_barrier_wait(a->bar);
+    /* owners apply / answer what was sent to them */
+    for (int src = 0; src < P; src++) {
+      int64_t *b = a->buf + ((int64_t)src * P + me) * kOwnBuf;
+      const int64_t c = a->cnt[(int64_t)src * P + me];
+      if (a->gather)
+        for (int64_t k = 0; k < c; k++) b[k] = a->table[b[k] * P + me];
+      else
+        for (int64_t k = 0; k < c; k++) a->counts[b[k] * P + me] += 1;
+    }
+    pthread_barrier_wait(a->bar);
+    if (a->gather) {
+      /* retrace this round's requests in order: each source's replies FIFO */
+      for (int d = 0; d < P; d++) pos[d] = 0;
+      for (int64_t k = i0; k < i; k++) {
+        const int d = (int)(a->index[k] % P);
+        a->tgt[k] = a->buf[((int64_t)me * P + d) * kOwnBuf + pos[d]++];
+      }
+      pthread_barrier_wait(a->bar);
+    }
+    int all = 1;
+    for (int q = 0; q < P; q++)
+      if (a->done[q] > round) all = 0;
+    if (all) break;
+  }
+  pthread_barrier_wait(a->bar);
+  a->secs = wall_seconds() - t0;
+  free(pos);
+  return NULL;
+}
+
+static double run_owner(const int64_t *index, int64_t n, int64_t *counts, int64_t *tgt,
+                        const int64_t *table, int P, int gather) {
+  if (P < 1) P = 1;
+  int64_t *buf = malloc(sizeof(int64_t) * (size_t)P * (size_t)P * kOwnBuf);
+  int64_t *cnt = calloc((size_t)P * (size_t)P, sizeof(int64_t));
+  int64_t *done = malloc(sizeof(int64_t) * (size_t)P);
+  pthread_t *th = malloc(sizeof(pthread_t) * (size_t)P);
+  own_arg *args = malloc(sizeof(own_arg) * (size_t)P);
+  if (!buf || !cnt || !done || !th || !args) {
+    free(buf); free(cnt); free(done); free(th); free(args);
+    return -1.0;
+  }
+  for (int q = 0; q < P; q++) done[q] = INT64_MAX;
+  pthread_barrier_t bar;
+  pthread_barrier_init(&bar, NULL, (unsigned)P);
+  for (int t = 0; t < P; t++) {
+    own_arg *g = &args[t];
+    g->index = index;
+    g->lo = n * t / P;
+    g->hi = n * (t + 1) / P;
+    g->counts = counts;
+    g->tgt = tgt;
+    g->table = table;
+    g->buf = buf;
+    g->cnt = cnt;
+    g->done = done;
+    g->bar = &bar;
+    g->me = t;
+    g->P = P;
+    g->gather = gather;
+    pthread_create(&th[t], NULL, own_body, g);
+  }
+  double sum = 0.0;
+  for (int t = 0; t < P; t++) {
+    pthread_join(th[t], NULL);
+    sum += args[t].secs;
+  }
+  pthread_barrier_destroy(&bar);
+  free(buf); free(cnt); free(done); free(th); free(args);
+  return sum / P;
+}
+
+double or_histo_owner_mt(const int64_t *index, int64_t n, int64_t *counts, int nthreads) {
+  return run_owner(index, n, counts, NULL, NULL, nthreads, 0);
+}
+
+double or_ig_owner_mt(int64_t *tgt, const int64_t *index, int64_t n, const int64_t *table,
+                      int nthreads) {
+  return run_owner(index, n, NULL, tgt, table, nthreads, 1);
+}
+
 /* ---- batched mix ------------------------------------------------------- */
 void or_batch_params(or_instance *out, int64_t count, uint64_t seed,
                      int lo_tab, int hi_tab, int lo_ops, int hi_ops) {

@@ -74,6 +74,15 @@ int or_histo_owner(const int64_t *pckindx, int64_t n, int64_t *lcounts,
  * threads of per-thread wall seconds between barriers
  * (libgetput/libgetput.upc:395-401). */
 double or_histo_agi_mt(const int64_t *index, int64_t n, int64_t *counts, int nthreads);
+/* Aggregate-by-owner with nthreads PEs (bin j owned by PE j % P), bulk
+ * synchronous rounds of at most 1024 items per destination as exstack runs them
+ * (apps/histo_src/histo_exstack.upc:64-76, histo_conveyor.upc:72-82); average
+ * per-PE seconds between barriers (libgetput/libgetput.upc:395-401). */
+double or_histo_owner_mt(const int64_t *index, int64_t n, int64_t *counts, int nthreads);
+/* The request/reply form (apps/ig_src/ig_exstack.upc:69-94): requests to the
+ * owner, replies in place, retraced in order. */
+double or_ig_owner_mt(int64_t *tgt, const int64_t *index, int64_t n, const int64_t *table,
+                      int nthreads);
 
 /* ---------------- index gather ---------------------------------------- */
 /* AGI model: tgt[i] = table[index[i]] (apps/ig_src/ig_agi.upc:61-64). */

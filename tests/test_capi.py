@@ -88,7 +88,7 @@ def test_host_side_argument_checks():
 @pytest.mark.parametrize("field,value,words", [
     ("strategy", 7, "strategy"), ("block", 300, "block"), ("unroll", 5, "unroll"),
     ("tile_log2", 40, "tile_log2"), ("merge", 3, "merge"), ("groups_per_cu", 99, "groups_per_cu"),
-    ("hot", 3, "hot"), ("task_log2", 5, "task_log2"),
+    ("hot", 3, "hot"), ("task_log2", 5, "task_log2"), ("task_log2", 31, "task_log2"),
 ])
 def test_configs_are_checked_on_the_host(field, value, words):
     """A configuration no shape can take is refused with BALE_ERR_ARG and a

@@ -77,16 +77,18 @@ def test_plain_gather_full(dev, orc):
     # the table against the torch restatement
     assert TC.gather_matches_splitmix(table, torch.arange(tlen, dtype=torch.int64, device=dev))
     tgt = torch.empty(n, dtype=torch.int64, device=dev)
-    for cfg in (None, B.config(B.DIRECT, block=1024, unroll=8), B.config(B.BUCKET, block=1024)):
+    for cfg in (None, B.config(B.DIRECT, block=1024, unroll=8), B.config(B.BUCKET, block=1024),
+                B.config(B.BUCKET, tile_log2=17, task_log2=10, groups_per_cu=2)):
         tgt.fill_(-7)
         B.gather(tgt, idx, table, cfg=cfg)
         assert TC.gather_matches_splitmix(tgt, idx), cfg
     # packed indices (the buffered models' input at one PE)
     pck = torch.empty_like(idx)
     B.pack(pck, idx)
-    tgt.fill_(-7)
-    B.gather(tgt, pck, table, packed=True)
-    assert TC.gather_matches_splitmix(tgt, idx)
+    for cfg in (None, B.config(B.BUCKET)):
+        tgt.fill_(-7)
+        B.gather(tgt, pck, table, packed=True, cfg=cfg)
+        assert TC.gather_matches_splitmix(tgt, idx), cfg
     del table, idx, tgt, pck
     torch.cuda.empty_cache()
 

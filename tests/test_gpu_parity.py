@@ -265,6 +265,58 @@ def test_gather_strategies_match_oracle(dev, orc, tlen, n, dist, s, run_mean):
             assert np.array_equal(to_np(tgt), want), (strat, blk, unroll, tile, packed)
 
 
+@pytest.mark.parametrize("dist,s,run_mean", [(0, 1.0, 0), (1, 1.1, 0), (1, 1.2, 64), (0, 1.0, 64)],
+                         ids=["uniform", "zipf1.1", "zipf_runs", "uniform_runs"])
+def test_gather_adaptive_plan(dev, orc, dist, s, run_mean):
+    """The planned gather for large shapes is BUCKET with a device-side sample
+    that hands inputs repeating entries to DIRECT: either way tgt equals the
+    oracle's ig_agi (apps/ig_src/ig_agi.upc:61-64), plain and packed."""
+    tlen, n = (1 << 22) + 3, 1 << 24
+    plan = B.gather_plan(n, tlen)
+    assert plan.strategy == B.BUCKET and plan.hot == 2
+    assert B.gather_plan(1 << 23, 1 << 26).strategy == B.DIRECT  # fewer requests than entries / 2
+    table = torch.empty(tlen, dtype=torch.int64, device=dev)
+    B.fill_splitmix(table)
+    idx = gen(dev, n, tlen, dist, s, run_mean, seed=31 + dist + run_mean)
+    want = orc.ig_agi(to_np(idx), orc.fill_splitmix(tlen))
+    pck = torch.empty_like(idx)
+    B.pack(pck, idx)
+    for packed, src in ((False, idx), (True, pck)):
+        tgt = torch.full((n,), 5, dtype=torch.int64, device=dev)
+        B.gather(tgt, src, table, packed=packed)
+        assert B.lib().bale_hip_last_strategy() == B.BUCKET
+        assert np.array_equal(to_np(tgt), want), packed
+
+
+def test_gather_bucket_errors_and_ragged(dev, orc):
+    """BUCKET gather: out-of-range and foreign-PE requests are counted and
+    their slots read 0; a ragged last chunk; regions narrower than default."""
+    tlen, n = 100_003, 3 * (1 << 14) + 77
+    table = torch.empty(tlen, dtype=torch.int64, device=dev)
+    B.fill_splitmix(table)
+    idx = gen(dev, n, tlen, seed=12)
+    bad = torch.arange(0, n, 97, device=dev)
+    idx[bad] = tlen + 11
+    want = orc.ig_agi(np.minimum(to_np(idx), tlen - 1), orc.fill_splitmix(tlen))
+    want[to_np(bad)] = 0
+    for tile in (0, 10, 12):
+        tgt = torch.full((n,), 5, dtype=torch.int64, device=dev)
+        with pytest.raises(B.BaleError) as e:
+            B.gather(tgt, idx, table, cfg=B.config(B.BUCKET, tile_log2=tile))
+        assert e.value.code == B.ERR_RANGE
+        assert np.array_equal(to_np(tgt), want), tile
+    pck = torch.empty_like(idx)
+    B.pack(pck, idx.clamp(max=tlen - 1))
+    pck[7] += 3  # PE 3
+    tgt = torch.full((n,), 5, dtype=torch.int64, device=dev)
+    with pytest.raises(B.BaleError) as e:
+        B.gather(tgt, pck, table, packed=True, cfg=B.config(B.BUCKET))
+    assert e.value.code == B.ERR_PE
+    w2 = orc.ig_agi(np.minimum(to_np(idx), tlen - 1), orc.fill_splitmix(tlen))
+    w2[7] = 0
+    assert np.array_equal(to_np(tgt), w2)
+
+
 def test_gather_closed_form_reference_table(dev, orc):
     tlen, n = 100_000, 1_000_000
     table = torch.empty(tlen, dtype=torch.int64, device=dev)

@@ -208,6 +208,19 @@ def test_histo_mt_matches(orc):
     assert np.array_equal(counts, np.bincount(idx, minlength=4096))
 
 
+@pytest.mark.parametrize("P", [1, 2, 5, 8])
+def test_histo_owner_mt_matches(orc, P):
+    """The threaded aggregate-by-owner baseline (bulk-synchronous rounds of at
+    most 1024 items per destination) gives numpy's bincount, including skewed
+    input whose hot owner fills a buffer every round."""
+    for idx in (orc.gen_uniform(300_000, 4099, 3), orc.gen_index(200_000, 5000, 1, 1.4, 0, 4),
+                np.zeros(5000, np.int64), np.zeros(0, np.int64)):
+        counts = np.full(5000, 2, np.int64)
+        assert orc.histo_owner_mt(idx, counts, P) >= 0
+        want = np.bincount(idx, minlength=5000) + 2
+        assert np.array_equal(counts, want), P
+
+
 # ---- index gather ------------------------------------------------------------
 def test_fill_closed_forms(orc):
     t = orc.fill_splitmix(1000)
@@ -248,6 +261,17 @@ def test_ig_mt_matches(orc):
     tgt = np.zeros_like(idx)
     orc.ig_agi_mt(idx, table, tgt, 4)
     assert np.array_equal(tgt, table[idx])
+
+
+@pytest.mark.parametrize("P", [1, 3, 8])
+def test_ig_owner_mt_matches(orc, P):
+    """The threaded request/reply baseline (requests to owners, replies in
+    place, retraced in order per round) gives table[index]."""
+    table = orc.fill_splitmix(50_001)
+    for idx in (orc.gen_uniform(200_000, 50_001, 8), orc.gen_index(120_000, 50_001, 1, 1.3, 16, 9)):
+        tgt = np.full_like(idx, -1)
+        assert orc.ig_owner_mt(idx, table, tgt, P) >= 0
+        assert np.array_equal(tgt, table[idx]), P
 
 
 # ---- batched mix parameters -----------------------------------------------------
