@@ -21,7 +21,7 @@ LIB = PKG / "libbale_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["capi.hip", "capi_spmat.hip", "spmat.hip", "toposort.hip", "gen.hip", "gather.hip", "histo.hip", "probe.hip", "batch.hip", "select.hip",
-               "convey.hip", "randperm.hip"]
+               "convey.hip", "exstack.hip", "randperm.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

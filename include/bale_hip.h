@@ -390,6 +390,61 @@ int bale_probe_random_read(const int64_t *table, int64_t n, int64_t nreads,
 int bale_probe_random_read_policy(const int64_t *table, int64_t n, int64_t nreads,
                                   int64_t *out, int policy, void *stream);
 
+/* ---- exstack and exstack2 ---------------------------------------------
+ * The aggregation buffers of exstack/exstack.h:94-107 (bulk synchronous) and
+ * :201-210 (asynchronous) as a bulk service on one GPU: items pushed with a
+ * source and a destination virtual PE (1..255 PEs, device arrays), delivered
+ * so that each destination receives its items grouped by source, each
+ * source's in push order.  A round holds up to n_pe * buf_cnt items in all
+ * (the reference bounds each destination's buffer by buf_cnt; a caller that
+ * pushes until a push is short and then exchanges behaves the same).  Spans
+ * handed out by the pops are device memory, valid until the next delivery.
+ * Negative returns are BALE_ERR_* codes (bale_hip_last_error() says why). */
+typedef struct bale_exstack bale_exstack;
+typedef struct bale_exstack bale_exstack2;
+/* exstack.h:94 exstack_init -- n_pe virtual PEs, buf_cnt items per PE per
+ * round, packages of pkg_size bytes (4, 8, 12, 16, 24 or 32).  NULL on error. */
+bale_exstack *bale_exstack_init(int64_t n_pe, int64_t buf_cnt, size_t pkg_size);
+/* exstack.h:95 exstack_proceed -- 0 once done is set, nothing is pushed and
+ * everything delivered was popped; else 1. */
+int64_t bale_exstack_proceed(bale_exstack *ex, int done);
+/* exstack.h:96 exstack_push, in bulk -- pushes the longest prefix of the n
+ * items (device arrays: packages, source PEs or NULL for PE 0, destination
+ * PEs) that fits this round, on `stream`; returns how many it took. */
+int64_t bale_exstack_push(bale_exstack *ex, const void *items, const int64_t *from_pes,
+                          const int64_t *to_pes, int64_t n, void *stream);
+/* exstack.h:97 exstack_exchange -- delivers the round (waits for `stream`);
+ * items naming a PE outside 0..n_pe-1 are dropped and reported as
+ * BALE_ERR_RANGE after the rest is delivered. */
+int bale_exstack_exchange(bale_exstack *ex, void *stream);
+/* exstack.h:100 exstack_pop, in bulk -- the items PE `pe` received this round
+ * (*items) and their source PEs (*from_pes, int64), grouped by source in
+ * increasing order; returns the count (0 once popped). */
+int64_t bale_exstack_pop(bale_exstack *ex, int64_t pe, const void **items, const int64_t **from_pes);
+/* exstack.h:98 exstack_pop_thread, in bulk -- what `from_pe` sent `pe` this
+ * round, in push order (the retrace of apps/ig_src/ig_exstack.upc:88-92);
+ * returns the count (0 once popped).  A PE's round is popped either whole or
+ * one source at a time. */
+int64_t bale_exstack_pop_thread(bale_exstack *ex, int64_t pe, int64_t from_pe, const void **items);
+/* exstack.h:106 exstack_reset: forget the round; :105 exstack_clear: free. */
+int bale_exstack_reset(bale_exstack *ex);
+void bale_exstack_clear(bale_exstack *ex);
+/* exstack.h:201 exstack2_init -- as bale_exstack_init. */
+bale_exstack2 *bale_exstack2_init(int64_t n_pe, int64_t buf_cnt, size_t pkg_size);
+/* exstack.h:203 exstack2_push, in bulk -- as bale_exstack_push. */
+int64_t bale_exstack2_push(bale_exstack2 *x, const void *items, const int64_t *from_pes,
+                           const int64_t *to_pes, int64_t n, void *stream);
+/* exstack.h:202 exstack2_proceed -- sends what was pushed once everything
+ * delivered before was popped (the asynchronous send of a full buffer), and
+ * returns 0 once done_pushing was given, nothing waits and everything
+ * delivered was popped; else 1. */
+int64_t bale_exstack2_proceed(bale_exstack2 *x, int done_pushing, void *stream);
+/* exstack.h:204 exstack2_pop, in bulk -- as bale_exstack_pop. */
+int64_t bale_exstack2_pop(bale_exstack2 *x, int64_t pe, const void **items, const int64_t **from_pes);
+/* exstack.h:209 exstack2_reset, :210 exstack2_clear. */
+int bale_exstack2_reset(bale_exstack2 *x);
+void bale_exstack2_clear(bale_exstack2 *x);
+
 /* ---- learned selector ----------------------------------------------------
  * 12 sketch features per instance and an fp32 MLP 12 -> 64 -> 64 -> 10
  * (3 strategy, 4 variant, 3 task-size logits); all arrays device memory. */
