@@ -118,6 +118,7 @@ def _proto(L) -> None:
     f("bale_batch_stats", C.c_int, VP, P64, C.c_int)
     f("bale_batch_destroy", None, VP)
     f("bale_batch_launch_plain", C.c_int, IP, I64T, VP)
+    f("bale_batch_launch_each", C.c_int, IP, I64T, VP)
     f("bale_tune", C.c_int, IP, I64T, I64T, C.POINTER(C.c_float), C.c_int, VP)
     f("bale_tune_candidates", C.c_int, C.c_int, I64T, C.POINTER(BaleConfig), C.c_int)
     f("bale_randperm", C.c_int, VP, I64T, I64T, VP)

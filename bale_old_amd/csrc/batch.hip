@@ -876,6 +876,27 @@ void bale_batch_destroy(bale_batch* b) {
   delete b;
 }
 
+int bale_batch_launch_each(const bale_instance* inst, int64_t count, void* stream) {
+  if (count < 0 || (count > 0 && !inst)) return bale_set_error(BALE_ERR_ARG, "bad instance array");
+  for (int64_t i = 0; i < count; i++) {
+    const bale_instance& in = inst[i];
+    if (check_instance(in) != BALE_OK)
+      return bale_set_error(BALE_ERR_ARG, "instance %lld: bad kind, size, pointer or dtype", (long long)i);
+    bale_config c = in.cfg;
+    int rc;
+    if (in.kind == 1) {
+      // a gather from an LDS copy of the table exists only inside batches
+      if (c.strategy == BALE_STRAT_LDS) c.strategy = BALE_STRAT_DIRECT;
+      rc = bale_gather(in.tgt, in.index, in.n, (const int64_t*)in.table, in.table_len,
+                       BALE_INDEX_PLAIN, &c, stream);
+    } else {
+      rc = bale_histo(in.index, in.n, in.table, in.table_len, in.dtype, BALE_INDEX_PLAIN, &c, stream);
+    }
+    if (rc) return rc;
+  }
+  return BALE_OK;
+}
+
 int bale_batch_launch_plain(const bale_instance* inst, int64_t count, void* stream) {
   if (count < 0 || (count > 0 && !inst)) return bale_set_error(BALE_ERR_ARG, "bad instance array");
   bale_config c;
@@ -931,6 +952,10 @@ int bale_tune_candidates(int kind, int64_t table_len, bale_config* out, int max)
   return n;
 }
 
+// Timed pairs per candidate (the median is kept; the reference's tuner also
+// averages repeated runs, convey/tune.pl:33-100).
+constexpr int kTunePairs = 3;
+
 int bale_tune(bale_instance* inst, int64_t count, int64_t min_ops, float* costs, int max_cand,
               void* stream) {
   if (count < 0 || (count > 0 && !inst)) return bale_set_error(BALE_ERR_ARG, "bad instance array");
@@ -940,11 +965,10 @@ int bale_tune(bale_instance* inst, int64_t count, int64_t min_ops, float* costs,
   ErrWords* err = stream_err(d, s);
   if (!err) return BALE_ERR_ARG;
   if (min_ops <= 0) min_ops = (int64_t)1 << 22;
-  Buffers bufs;
+  Buffers bufs, bufs2;
   DevBuf scratch;
-  hipEvent_t e0, e1;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
+  hipEvent_t ev[4 * kTunePairs];
+  for (hipEvent_t& e : ev) (void)hipEventCreate(&e);
   std::vector<bale_config> cand(64);
   int rc = BALE_OK;
   for (int64_t i = 0; i < count && rc == BALE_OK; i++) {
@@ -967,46 +991,60 @@ int bale_tune(bale_instance* inst, int64_t count, int64_t min_ops, float* costs,
     int64_t R = std::max<int64_t>(1, std::min<int64_t>(256, ceil_div(min_ops, in.n)));
     float best = 3.4e38f;
     int bestk = 0;
-    // Seconds of one run of `copies` copies of candidate t as a batch (after a
-    // warm-up run); < 0 when the plan does not take it, -2 on a failed run.
-    auto time_copies = [&](const bale_instance& t, int64_t copies) -> float {
-      Plan P;
-      for (int64_t r = 0; r < copies; r++)
-        if (plan_add(P, t, d) != BALE_OK) return -1.f;
-      if (plan_upload(P, bufs, s)) {
+    // The marginal cost of candidate t: in a large batch a strategy's phases
+    // run anyway (launches, sketches, the bucketed plan's scans), so the cost
+    // an instance adds is the difference between R and 2R copies, not R copies
+    // alone.  Both plans are built, uploaded and warmed up once, then timed in
+    // kTunePairs interleaved pairs; the cost is the median of the pairs'
+    // (t(2R) - t(R)) / (R n), each floored at a tenth of the pair's average so
+    // noise cannot make it 0.  < 0 when the plan does not take the candidate,
+    // -2 on a failed run.
+    auto marginal = [&](const bale_instance& t) -> float {
+      Plan P1, P2;
+      for (int64_t r = 0; r < R; r++)
+        if (plan_add(P1, t, d) != BALE_OK) return -1.f;
+      for (int64_t r = 0; r < 2 * R; r++)
+        if (plan_add(P2, t, d) != BALE_OK) return -1.f;
+      if (plan_upload(P1, bufs, s) || plan_upload(P2, bufs2, s)) {
         rc = bale_set_error(BALE_ERR_ALLOC, "tuner upload");
         return -2.f;
       }
-      set_batch_lds(P);
-      plan_enqueue(P, bufs, d, s, err);  // warm-up
-      (void)hipEventRecord(e0, s);
-      plan_enqueue(P, bufs, d, s, err);
-      (void)hipEventRecord(e1, s);
-      if (hipEventSynchronize(e1) != hipSuccess) {
+      set_batch_lds(P1);
+      set_batch_lds(P2);
+      plan_enqueue(P1, bufs, d, s, err);  // warm-ups
+      plan_enqueue(P2, bufs2, d, s, err);
+      for (int k = 0; k < kTunePairs; k++) {
+        (void)hipEventRecord(ev[4 * k], s);
+        plan_enqueue(P1, bufs, d, s, err);
+        (void)hipEventRecord(ev[4 * k + 1], s);
+        (void)hipEventRecord(ev[4 * k + 2], s);
+        plan_enqueue(P2, bufs2, d, s, err);
+        (void)hipEventRecord(ev[4 * k + 3], s);
+      }
+      if (hipEventSynchronize(ev[4 * kTunePairs - 1]) != hipSuccess) {
         rc = bale_set_error(BALE_ERR_HIP, "tuner run failed");
         return -2.f;
       }
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      return ms * 1e-3f;
+      float c[kTunePairs];
+      for (int k = 0; k < kTunePairs; k++) {
+        float m1 = 0.f, m2 = 0.f;
+        (void)hipEventElapsedTime(&m1, ev[4 * k], ev[4 * k + 1]);
+        (void)hipEventElapsedTime(&m2, ev[4 * k + 2], ev[4 * k + 3]);
+        const float t1 = m1 * 1e-3f, t2 = m2 * 1e-3f;
+        const float avg = t2 / (float)(2 * R * in.n);
+        c[k] = std::max((t2 - t1) / (float)(R * in.n), 0.1f * avg);
+      }
+      std::sort(c, c + kTunePairs);
+      return c[kTunePairs / 2];
     };
     for (int k = 0; k < nc; k++) {
       bale_instance t = in;
       t.cfg = cand[k];
       if (in.kind == 1) t.tgt = (int64_t*)scratch.p;
       else t.table = scratch.p;
-      // The marginal cost: in a large batch a strategy's phases run anyway
-      // (launches, sketches, the bucketed plan's scans), so the cost an
-      // instance adds is the difference between R and 2R copies, not R copies
-      // alone; floored at a tenth of the average so noise cannot make it 0.
-      const float t1 = time_copies(t, R);
-      if (t1 == -2.f) break;
-      if (t1 < 0.f) continue;
-      const float t2 = time_copies(t, 2 * R);
-      if (t2 == -2.f) break;
-      if (t2 < 0.f) continue;
-      const float avg = t2 / (float)(2 * R * in.n);
-      const float cost = std::max((t2 - t1) / (float)(R * in.n), 0.1f * avg);
+      const float cost = marginal(t);
+      if (cost == -2.f) break;
+      if (cost < 0.f) continue;
       if (costs && k < max_cand) costs[i * max_cand + k] = cost;
       if (cost < best) {
         best = cost;
@@ -1015,8 +1053,7 @@ int bale_tune(bale_instance* inst, int64_t count, int64_t min_ops, float* costs,
     }
     in.cfg = cand[bestk];
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
   // the tuner's runs counted into scratch: clear the device error record
   int64_t bad = 0;
   (void)bale_sync_errors(stream, &bad);

@@ -158,6 +158,12 @@ def launch_plain(inst, count: int, stream=None) -> None:
     check(lib().bale_batch_launch_plain(inst, count, _stream(stream)))
 
 
+def launch_each(inst, count: int, stream=None) -> None:
+    """Each instance's own configuration as one library call per instance."""
+    from . import _stream
+    check(lib().bale_batch_launch_each(inst, count, _stream(stream)))
+
+
 def tune(inst, count: int, min_ops: int = 1 << 22, max_cand: int = 32, stream=None):
     """Exhaustive autotuner: fills inst[i].cfg; returns the cost table
     (count x max_cand seconds per op, -1 where unused) as a list of lists."""

@@ -2,6 +2,12 @@
 
     python -m bale_old_amd.train_selector --instances 20000 --out bale_old_amd/selector_weights.bin
 
+or in parts (each part a whole number of chunks; the parts' seeds follow on):
+
+    python -m bale_old_amd.train_selector --instances 10240 --collect-only --dump a.pt
+    python -m bale_old_amd.train_selector --instances 9760 --seed0 1020 --collect-only --dump b.pt
+    python -m bale_old_amd.train_selector --from-dump a.pt b.pt
+
 Training instances come from mixes with seeds 1000, 1001, ... (disjoint from
 the benchmark mixes, seeds 4 and 5), generated and tuned in chunks so that at
 most one chunk's tables are resident.  For each instance the sketch features
@@ -56,14 +62,18 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--instances", type=int, default=20000)
     ap.add_argument("--chunk", type=int, default=512)
-    ap.add_argument("--min-ops", type=int, default=1 << 20)
+    ap.add_argument("--min-ops", type=int, default=1 << 19)
+    ap.add_argument("--seed0", type=int, default=1000,
+                    help="first mix seed (collect in parts: a part of k chunks ends before seed0 + k)")
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=str(S.WEIGHTS))
     ap.add_argument("--dump", default=None, help="also save features/labels/costs (torch.save)")
-    ap.add_argument("--from-dump", default=None,
-                    help="train from a --dump file instead of collecting (same seeds and chunks)")
+    ap.add_argument("--from-dump", nargs="+", default=None,
+                    help="train from --dump files instead of collecting (parts in seed order, "
+                         "together the seeds from 1000 in chunks of --chunk)")
+    ap.add_argument("--collect-only", action="store_true", help="collect and --dump, do not train")
     ap.add_argument("--loss", choices=("cost", "ce"), default="cost")
     a = ap.parse_args(argv)
 
@@ -73,8 +83,11 @@ def main(argv=None) -> int:
     dev = torch.device("cuda:0")
     lib().bale_hip_init(0)
     if a.from_dump:
-        d = torch.load(a.from_dump, weights_only=True)
-        feats, labels, costs, kinds = d["features"], d["labels"], d["costs"], d["kinds"]
+        parts = [torch.load(f, weights_only=True) for f in a.from_dump]
+        feats = torch.cat([d["features"] for d in parts])
+        labels = torch.cat([d["labels"] for d in parts])
+        costs = [c for d in parts for c in d["costs"]]
+        kinds = [k for d in parts for k in d["kinds"]]
         params, seed = [], 1000
         while len(params) < feats.shape[0]:
             params += M.mix_params(min(a.chunk, feats.shape[0] - len(params)), seed)
@@ -84,7 +97,13 @@ def main(argv=None) -> int:
         tlens = [p.table_len for p in params]
         nops = [p.nops for p in params]
     else:
-        feats, labels, costs, kinds, tlens, nops = collect(a.instances, a.chunk, dev, a.min_ops, log=log)
+        feats, labels, costs, kinds, tlens, nops = collect(a.instances, a.chunk, dev, a.min_ops,
+                                                           seed0=a.seed0, log=log)
+        if a.collect_only:
+            torch.save({"features": feats, "labels": labels, "costs": costs, "kinds": kinds,
+                        "tlens": tlens, "nops": nops}, a.dump)
+            log(f"[train] saved {feats.shape[0]} instances to {a.dump}")
+            return 0
     n = feats.shape[0]
     n_val = max(1, n // 10)
     tr_f, tr_l = feats[n_val:].to(dev), labels[n_val:].to(dev)

@@ -131,23 +131,23 @@ def time_calls(fn, reps, stream):
     return median(ts)
 
 
-def check_mix_sample(mix, runs, k=48):
-    """Sampled exactness check of a mix after `runs` applications: gathers
-    against splitmix64 of the index, histograms against numpy.bincount."""
-    import numpy as np
-    step = max(1, len(mix.params) // k)
+def check_mix_sample(mix, runs, k=None):
+    """Exactness of every instance of a mix after `runs` applications, on the
+    GPU: gathers against splitmix64 of the index (restated with torch integer
+    arithmetic), histograms against torch.bincount.  k: only every
+    (count / k)-th instance."""
+    import torch
+    sys.path.insert(0, str(ROOT / "tests"))
+    from torch_check import splitmix64 as sm64
+    step = max(1, len(mix.params) // k) if k else 1
     for i in range(0, len(mix.params), step):
         p = mix.params[i]
         tv, iv, gv = mix.views[i]
-        idx = iv.cpu().numpy()
         if p.kind == 1:
-            got = gv.cpu().numpy()
-            sel = np.arange(0, idx.size, max(1, idx.size // 512))
-            for s in sel:
-                assert (int(got[s]) & M64) == splitmix64(int(idx[s])), ("gather", i)
+            assert torch.equal(gv, sm64(iv)), ("gather", i)
         else:
-            want = np.bincount(idx, minlength=p.table_len) * runs
-            assert np.array_equal(tv.cpu().numpy(), want), ("histo", i)
+            want = torch.bincount(iv, minlength=p.table_len) * runs
+            assert torch.equal(tv.to(torch.int64), want), ("histo", i)
 
 
 def bench_batch(B, torch, dev, stream, seed, count, reps, min_ops, learned):
@@ -185,15 +185,50 @@ def bench_batch(B, torch, dev, stream, seed, count, reps, min_ops, learned):
     out["exhaustive_ms"] = t_ex * 1e3
     out["exhaustive_GUPS"] = ops / t_ex / 1e9
     out["exhaustive_vs_plain"] = t_plain / t_ex
+    # what batching alone buys and what tuning alone buys: the tuned choices as
+    # one library call per instance; the library's default rule and DIRECT
+    # everywhere, each in one graph like the tuned batch
+    mix.reset_counts()
+    t_each = time_calls(lambda: MX.launch_each(mix.inst, count, stream), reps, stream)
+    B.sync_errors(stream)
+    check_mix_sample(mix, reps + 1)
+    out["tuned_unbatched_ms"] = t_each * 1e3
+    out["tuned_unbatched_GUPS"] = ops / t_each / 1e9
+    for name, strat in (("default_rule_graph", B.AUTO), ("direct_graph", B.DIRECT)):
+        mix.reset_counts()
+        mix.set_configs([B.config(strat) for _ in params])
+        bg = MX.Batch(mix.inst, count, stream)
+        t_g = time_calls(lambda: bg.launch(stream), reps, stream)
+        B.sync_errors(stream)
+        check_mix_sample(mix, reps + 1)
+        bg.close()
+        out[f"{name}_ms"] = t_g * 1e3
+        out[f"{name}_GUPS"] = ops / t_g / 1e9
+        out[f"exhaustive_vs_{name}"] = t_g / t_ex
+    mix.set_configs(tuned)
+    # how often tuning the same instances again picks the same candidate (the
+    # labels' own noise, an upper bound for any selector's agreement)
+    sub = list(range(0, count, max(1, count // 256)))
+    sub_inst = (MX.Instance * len(sub))(*[mix.inst[i] for i in sub])
+    MX.tune(sub_inst, len(sub), min_ops=min_ops, stream=stream)
+    torch.cuda.synchronize()
+    again = [MX.BaleConfigCopy(sub_inst[k].cfg) for k in range(len(sub))]
+    out["retune_agreement"] = {"instances": len(sub),
+                               "all_heads": sum(S.labels_of(again[k]) == S.labels_of(tuned[i])
+                                                for k, i in enumerate(sub)) / len(sub),
+                               "per_head": [sum(S.labels_of(again[k])[h] == S.labels_of(tuned[i])[h]
+                                                for k, i in enumerate(sub)) / len(sub) for h in range(3)]}
     if learned is not None:
         mix.reset_counts()
         t0 = time.time()
         S.select(mix.inst, count, learned, dev, stream)
         torch.cuda.synchronize()
         out["select_seconds"] = time.time() - t0
-        agree = sum(1 for a, b in zip(tuned, mix.configs())
-                    if S.labels_of(a) == S.labels_of(b))
+        got = mix.configs()
+        agree = sum(1 for a, b in zip(tuned, got) if S.labels_of(a) == S.labels_of(b))
         out["learned_agrees_with_exhaustive"] = agree / count
+        out["learned_head_agreement"] = [sum(S.labels_of(a)[h] == S.labels_of(b)[h]
+                                             for a, b in zip(tuned, got)) / count for h in range(3)]
         bl = MX.Batch(mix.inst, count, stream)
         t_l = time_calls(lambda: bl.launch(stream), reps, stream)
         B.sync_errors(stream)
@@ -652,11 +687,11 @@ def main() -> int:
             extra["selector_training"] = meta
         log("batched mix, seed 4 (plain vs exhaustive) ...")
         extra["batch_mix_seed4"] = bench_batch(B, torch, dev, stream, 4, args.batch_instances, 3,
-                                               1 << 20, None)
+                                               1 << 19, None)
         log(f"seed 4: {extra['batch_mix_seed4']}")
         log("held-out mix, seed 5 (learned vs exhaustive vs plain) ...")
         extra["batch_mix_seed5"] = bench_batch(B, torch, dev, stream, 5, args.batch_instances, 3,
-                                               1 << 20, learned)
+                                               1 << 19, learned)
         log(f"seed 5: {extra['batch_mix_seed5']}")
 
     out = {

@@ -365,12 +365,16 @@ int bale_batch_stats(const bale_batch *b, int64_t *out, int nout);
 void bale_batch_destroy(bale_batch *b);
 /* Baseline: one plain DIRECT kernel launch per instance, in order, on `stream`. */
 int bale_batch_launch_plain(const bale_instance *inst, int64_t count, void *stream);
+/* Baseline: each instance with its own cfg as one bale_histo / bale_gather call,
+ * in order, on `stream` (what tuning buys without the batched launch). */
+int bale_batch_launch_each(const bale_instance *inst, int64_t count, void *stream);
 
 /* Exhaustive autotuner: time every candidate configuration of each instance
  * (as batches of that instance repeated R and 2R times, R x ops about
  * min_ops, results in scratch so caller data is untouched) and store the one
  * with the lowest marginal cost in inst[i].cfg: the time the R extra copies
- * add per operation, which is what an instance costs inside a large batch.
+ * add per operation, which is what an instance costs inside a large batch,
+ * as the median of three interleaved R / 2R pairs.
  * costs (optional): count x max_cand seconds per operation, -1 where unused. */
 int bale_tune(bale_instance *inst, int64_t count, int64_t min_ops, float *costs, int max_cand,
               void *stream);

@@ -172,3 +172,29 @@ def test_batched_mix_full(dev, orc, seed):
     _check_mix_full(mix, 4)
     del mix
     torch.cuda.empty_cache()
+
+
+@pytest.mark.gpu
+def test_batched_mix_tuned_full(dev):
+    """The seed-4 mix with the exhaustive tuner's own choices: every 8th
+    instance tuned (the whole mix takes minutes; the bench tunes all of it and
+    checks every instance the same way), the rest on the default rule, one
+    graph launched twice, every instance checked."""
+    from bale_old_amd import mix as MX
+    params = MX.mix_params(4096, 4)
+    mix = MX.Mix(params, dev)
+    sub = list(range(0, 4096, 8))
+    sub_inst = (MX.Instance * len(sub))(*[mix.inst[i] for i in sub])
+    MX.tune(sub_inst, len(sub), min_ops=1 << 17)
+    for k, i in enumerate(sub):
+        mix.inst[i].cfg = sub_inst[k].cfg
+    kinds = {(c.strategy, c.hot, c.merge > 0) for i, c in enumerate(mix.configs()) if i % 8 == 0}
+    assert len(kinds) >= 3  # the tuner picks several strategies on this mix
+    b = MX.Batch(mix.inst, 4096)
+    b.launch()
+    b.launch()
+    B.sync_errors()
+    b.close()
+    _check_mix_full(mix, 2)
+    del mix
+    torch.cuda.empty_cache()
