@@ -122,8 +122,9 @@ def _proto(L) -> None:
     f("bale_tune", C.c_int, IP, I64T, I64T, C.POINTER(C.c_float), C.c_int, VP)
     f("bale_tune_candidates", C.c_int, C.c_int, I64T, C.POINTER(BaleConfig), C.c_int)
     f("bale_randperm", C.c_int, VP, I64T, I64T, VP)
+    f("bale_randperm_shuffle", C.c_int, VP, I64T, I64T, VP)
     f("bale_is_perm", C.c_int, VP, I64T, VP)
-    for name in ("rand_permp_agi", "rand_permp_conveyor"):
+    for name in ("rand_permp_agi", "rand_permp_conveyor", "rand_permp_agi_opt"):
         f(name, VP, I64T, C.c_int)
     for name in ("rand_permp_exstack", "rand_permp_exstack2"):
         f(name, VP, I64T, C.c_int, I64T)
@@ -286,6 +287,13 @@ def randperm(perm, seed: int, stream=None) -> None:
     """perm (int64 device tensor of length n) = the dart-throwing random
     permutation of 0..n-1 (rand_permp_agi at one PE, spmat/spmat_agi.upc:56)."""
     check(lib().bale_randperm(_ptr(perm), perm.numel(), seed, _stream(stream)))
+
+
+def randperm_shuffle(perm, seed: int, stream=None) -> None:
+    """perm (int64 device tensor of length n) = the Fisher-Yates permutation
+    of 0..n-1 (rand_permp_agi_opt at one PE,
+    apps/randperm_src/alternates/randperm_agi_opt.upc)."""
+    check(lib().bale_randperm_shuffle(_ptr(perm), perm.numel(), seed, _stream(stream)))
 
 
 def is_perm(perm, stream=None) -> bool:

@@ -65,10 +65,13 @@ def build_lib(jobs: int = 8) -> Path:
 
 
 def build_host() -> list[Path]:
-    """C drivers mirroring the reference's programs."""
+    """C drivers mirroring the reference's programs, under the reference's
+    binary names (apps/Makefile.am:39)."""
     out = []
-    for prog in ("histo", "ig", "randperm", "transpose_matrix", "permute_matrix", "toposort"):
-        src = HOST / f"{prog}.c"
+    progs = {"histo": "histo", "ig": "ig", "randperm": "randperm", "transpose_matrix": "transpose_matrix",
+             "permute_matrix": "permute_matrix", "topo": "toposort"}
+    for prog, stem in progs.items():
+        src = HOST / f"{stem}.c"
         if not src.exists():
             continue
         exe = BUILD / prog

@@ -154,6 +154,8 @@ int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbi
 // receives the darts generated
 int launch_randperm(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream_t s,
                     int64_t* darts_used);
+// random permutation by a Fisher-Yates pass resolved in parallel (randperm.hip)
+int launch_randperm_shuffle(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream_t s);
 // *ok = perm[0..N) holds 0..N-1 once each (waits for the stream)
 int launch_check_perm(const int64_t* perm, int64_t N, Device* d, hipStream_t s, int* ok);
 
@@ -180,7 +182,5 @@ int launch_count_diff(const int64_t* a, const int64_t* b, int64_t n, Device* d, 
 int launch_probe_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 int launch_probe_random(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
                         hipStream_t s);
-int launch_probe_random_policy(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
-                               int policy, hipStream_t s);
 
 }  // namespace bale

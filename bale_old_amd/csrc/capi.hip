@@ -608,13 +608,6 @@ int bale_probe_random_read(const int64_t* table, int64_t n, int64_t nreads, int6
   return rc ? fail_hip(rc, "probe_random") : BALE_OK;
 }
 
-int bale_probe_random_read_policy(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
-                                  int policy, void* stream) {
-  if (n <= 0 || nreads < 0 || !table || !out) return fail(BALE_ERR_ARG, "bad probe arguments");
-  int rc = launch_probe_random_policy(table, n, nreads, out, policy, (hipStream_t)stream);
-  return rc ? fail_hip(rc, "probe_random_policy") : BALE_OK;
-}
-
 }  // extern "C"
 
 // ======================= reference drop-ins ===============================

@@ -25,6 +25,18 @@ int bale_randperm(int64_t* perm, int64_t n, int64_t seed, void* stream) {
   return rc ? bale_fail_hip(rc, "randperm") : BALE_OK;
 }
 
+int bale_randperm_shuffle(int64_t* perm, int64_t n, int64_t seed, void* stream) {
+  if (n < 0) return bale_set_error(BALE_ERR_ARG, "negative permutation length %lld", (long long)n);
+  if (n == 0) return BALE_OK;
+  if (!perm) return bale_set_error(BALE_ERR_ARG, "null permutation");
+  if (n >= ((int64_t)1 << 30)) return bale_set_error(BALE_ERR_ARG, "permutation length must be below 2^30");
+  Device* d = device_state();
+  if (!d) return bale_set_error(BALE_ERR_NODEV, "no usable HIP device");
+  int rc = launch_randperm_shuffle(perm, n, seed, d, (hipStream_t)stream);
+  if (rc == hipErrorOutOfMemory) return bale_set_error(BALE_ERR_ALLOC, "workspace allocation failed");
+  return rc ? bale_fail_hip(rc, "randperm_shuffle") : BALE_OK;
+}
+
 int bale_is_perm(const int64_t* perm, int64_t n, void* stream) {
   if (n < 0 || (n > 0 && !perm)) return bale_set_error(BALE_ERR_ARG, "bad permutation array");
   Device* d = device_state();
@@ -41,7 +53,7 @@ int bale_is_perm(const int64_t* perm, int64_t n, void* stream) {
 // bale_all_alloc, filled by the dart-throwing AGI model (the result the
 // reference's rand_permp_agi gives at one PE), and the GPU seconds go to
 // bale_hip_last_timing.  NULL on error.
-static int64_t* randperm_dropin(int64_t N, int seed) {
+static int64_t* randperm_dropin(int64_t N, int seed, bool shuffle = false) {
   set_last_timing(0.0, 0.0, 0.0);
   if (N < 0 || N >= ((int64_t)1 << 30)) {
     bale_set_error(BALE_ERR_ARG, "permutation length %lld outside [0, 2^30)", (long long)N);
@@ -54,7 +66,7 @@ static int64_t* randperm_dropin(int64_t N, int seed) {
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   hipEventRecord(e0, 0);
-  int rc = N ? launch_randperm(perm, N, seed, d, 0, nullptr) : 0;
+  int rc = !N ? 0 : shuffle ? launch_randperm_shuffle(perm, N, seed, d, 0) : launch_randperm(perm, N, seed, d, 0, nullptr);
   hipEventRecord(e1, 0);
   if (!rc) rc = (int)hipEventSynchronize(e1);
   if (!rc) set_last_timing(elapsed(e0, e1), 0.0, 0.0);
@@ -75,6 +87,10 @@ int64_t* rand_permp_exstack2(int64_t N, int seed, int64_t) { return randperm_dro
 int64_t* rand_permp_conveyor(int64_t N, int seed) { return randperm_dropin(N, seed); }
 
 int64_t* rand_permp(int64_t N, int seed) { return randperm_dropin(N, seed); }
+
+// The alternate model: the Fisher-Yates pass, the result the reference's
+// rand_permp_agi_opt gives at one PE.
+int64_t* rand_permp_agi_opt(int64_t N, int seed) { return randperm_dropin(N, seed, true); }
 
 // ---- sparse matrices: native API ------------------------------------------------
 

@@ -31,53 +31,6 @@ __global__ __launch_bounds__(256) void k_probe_random(const int64_t* __restrict_
   if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)acc);
 }
 
-// Random reads through a buffer descriptor with cache policy POLICY on every
-// table load (bit 0 sc0, bit 1 nt, bit 4 sc1).  The descriptor's record
-// count is the table size, so a stray offset reads 0 instead of faulting.
-template <int POLICY>
-__global__ __launch_bounds__(256) void k_probe_random_buf(const int64_t* __restrict__ table,
-                                                          uint64_t n, int64_t nreads,
-                                                          unsigned long long* __restrict__ out) {
-  constexpr int U = 16;
-  __amdgpu_buffer_rsrc_t rs =
-      __builtin_amdgcn_make_buffer_rsrc((void*)table, (short)0, (int)(uint32_t)(n * 8), 0x00020000);
-  int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * U;
-  uint64_t v[U];
-#pragma unroll
-  for (int k = 0; k < U; k++) {
-    int64_t r = base + k;
-    uint32_t off = (uint32_t)(reduce_range(mix64((uint64_t)r), n) * 8);
-    v[k] = r < nreads ? (uint64_t)__builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, POLICY) : 0;
-  }
-  uint64_t acc = 0;
-#pragma unroll
-  for (int k = 0; k < U; k++) acc += v[k];
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)acc);
-}
-
-int launch_probe_random_policy(const int64_t* table, int64_t n, int64_t nreads, int64_t* out,
-                               int policy, hipStream_t s) {
-  if (nreads <= 0 || n <= 0) return 0;
-  if (n * 8 > (int64_t)0xffffffff) return (int)hipErrorInvalidValue;  // 32-bit buffer offsets
-  int64_t grid = ceil_div(nreads, (int64_t)256 * 16);
-  if (grid > 0x7fffffff) return (int)hipErrorInvalidValue;
-  dim3 g((unsigned)grid), b(256);
-  unsigned long long* o = (unsigned long long*)out;
-  switch (policy) {
-    case 0: hipLaunchKernelGGL(k_probe_random_buf<0>, g, b, 0, s, table, (uint64_t)n, nreads, o); break;
-    case 1: hipLaunchKernelGGL(k_probe_random_buf<1>, g, b, 0, s, table, (uint64_t)n, nreads, o); break;
-    case 2: hipLaunchKernelGGL(k_probe_random_buf<2>, g, b, 0, s, table, (uint64_t)n, nreads, o); break;
-    case 3: hipLaunchKernelGGL(k_probe_random_buf<3>, g, b, 0, s, table, (uint64_t)n, nreads, o); break;
-    case 16: hipLaunchKernelGGL(k_probe_random_buf<16>, g, b, 0, s, table, (uint64_t)n, nreads, o); break;
-    case 17: hipLaunchKernelGGL(k_probe_random_buf<17>, g, b, 0, s, table, (uint64_t)n, nreads, o); break;
-    case 18: hipLaunchKernelGGL(k_probe_random_buf<18>, g, b, 0, s, table, (uint64_t)n, nreads, o); break;
-    case 19: hipLaunchKernelGGL(k_probe_random_buf<19>, g, b, 0, s, table, (uint64_t)n, nreads, o); break;
-    default: return (int)hipErrorInvalidValue;
-  }
-  return (int)hipGetLastError();
-}
-
 int launch_probe_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
   int64_t nvec = bytes / 16;
   if (nvec <= 0) return 0;

@@ -1,5 +1,6 @@
 // randperm.hip -- random permutation of 0..N-1 by dart throwing
-// (spmat/spmat_agi.upc:56-107, rand_permp_agi).
+// (spmat/spmat_agi.upc:56-107, rand_permp_agi), and by a Fisher-Yates pass
+// (apps/randperm_src/alternates/randperm_agi_opt.upc, rand_permp_agi_opt).
 //
 // At one PE the reference is sequential: darts r = lrand48() % M (M = 2N) are
 // thrown in turn, and the i-th dart that finds its slot empty places item i;
@@ -224,7 +225,101 @@ __global__ __launch_bounds__(256) void k_check_perm(const int64_t* __restrict__ 
   }
 }
 
+// ---- Fisher-Yates over 0..N-1 (rand_permp_agi_opt at one PE) -------------
+// Step i swaps positions i and j_i >= i.  Position i is final after step i,
+// and a position p >= i has been written only by steps k < i with j_k = p, the
+// last of which left there what position k held when step k ran.  So with
+//   pred(k) = the last step before k whose target is k (none: k untouched),
+//   W(k)    = what position k holds when step k runs
+//           = W(pred(k)), or k when there is no pred(k),
+//   prev(i) = the last step before i with the same target j_i,
+// the result is perm[i] = W(prev(i)), or j_i when there is no prev(i).  pred
+// strictly decreases, so W is a finite chase.  The steps sharing a target are
+// linked into a list per target (one atomic exchange each), and each step
+// finds prev and pred by walking two such lists; a target's list holds about
+// ln(N / (N - p)) steps, fewer than one on average.
+
+// j[i] = i + (lrand48 value N + i) mod (N - i): the first N values choose the
+// (only) PE and are skipped.
+__global__ __launch_bounds__(256) void k_fy_targets(uint64_t x0, uint32_t N, uint32_t* __restrict__ j) {
+  const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kDartsPerThread;
+  if (i0 >= N) return;
+  const Lcg f = lcg_steps((uint64_t)N + i0);
+  uint64_t x = (f.a * x0 + f.c) & kMask48;
+  for (int t = 0; t < kDartsPerThread && i0 + t < N; t++) {
+    x = (kA * x + kC) & kMask48;
+    const uint32_t i = (uint32_t)(i0 + t);
+    j[i] = i + (uint32_t)(x >> 17) % (N - i);
+  }
+}
+
+// next[i] = the previous head of target j_i's list; head[j_i] = i.
+__global__ __launch_bounds__(256) void k_fy_link(const uint32_t* __restrict__ j, uint32_t N,
+                                                 uint32_t* __restrict__ head, uint32_t* __restrict__ next) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+    next[i] = __hip_atomic_exchange(head + j[i], i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The largest member below `below` of the list starting at q (kEmpty if none).
+__device__ inline uint32_t fy_last_below(uint32_t q, uint32_t below, const uint32_t* __restrict__ next) {
+  uint32_t best = kEmpty;
+  while (q != kEmpty) {
+    if (q < below && (best == kEmpty || q > best)) best = q;
+    q = next[q];
+  }
+  return best;
+}
+
+__global__ __launch_bounds__(256) void k_fy_walk(const uint32_t* __restrict__ j, uint32_t N,
+                                                 const uint32_t* __restrict__ head,
+                                                 const uint32_t* __restrict__ next,
+                                                 uint32_t* __restrict__ prev, uint32_t* __restrict__ pred) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    prev[i] = fy_last_below(head[j[i]], i, next);
+    pred[i] = fy_last_below(head[i], i, next);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fy_place(const uint32_t* __restrict__ j, uint32_t N,
+                                                  const uint32_t* __restrict__ prev,
+                                                  const uint32_t* __restrict__ pred,
+                                                  int64_t* __restrict__ perm) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    uint32_t q = prev[i];
+    if (q == kEmpty) {
+      perm[i] = (int64_t)j[i];
+      continue;
+    }
+    for (uint32_t r = pred[q]; r != kEmpty; r = pred[q]) q = r;
+    perm[i] = (int64_t)q;
+  }
+}
+
 }  // namespace rp
+
+int launch_randperm_shuffle(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream_t s) {
+  using namespace rp;
+  if (N <= 0) return 0;
+  const uint64_t n = (uint64_t)N, b = ((n * 4 + 255) & ~(uint64_t)255);
+  char* ws = (char*)workspace(d, 5 * b, s);
+  if (!ws) return (int)hipErrorOutOfMemory;
+  uint32_t* j = (uint32_t*)ws;
+  uint32_t* head = (uint32_t*)(ws + b);
+  uint32_t* next = (uint32_t*)(ws + 2 * b);
+  uint32_t* prev = (uint32_t*)(ws + 3 * b);
+  uint32_t* pred = (uint32_t*)(ws + 4 * b);
+  const uint64_t x0 = seed != 0 ? ((uint64_t)(uint32_t)seed << 16) | 0x330EULL : 0x1234ABCD330EULL;
+  const unsigned g = (unsigned)std::min<uint64_t>((uint64_t)d->cus * 16, (n + 255) / 256);
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::min(g, 4096u)), dim3(256), 0, s, head, n, kEmpty);
+  const uint64_t nthr = (n + kDartsPerThread - 1) / kDartsPerThread;
+  hipLaunchKernelGGL(k_fy_targets, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, x0, (uint32_t)N, j);
+  hipLaunchKernelGGL(k_fy_link, dim3(g), dim3(256), 0, s, (const uint32_t*)j, (uint32_t)N, head, next);
+  hipLaunchKernelGGL(k_fy_walk, dim3(g), dim3(256), 0, s, (const uint32_t*)j, (uint32_t)N,
+                     (const uint32_t*)head, (const uint32_t*)next, prev, pred);
+  hipLaunchKernelGGL(k_fy_place, dim3(g), dim3(256), 0, s, (const uint32_t*)j, (uint32_t)N,
+                     (const uint32_t*)prev, (const uint32_t*)pred, perm);
+  return (int)hipGetLastError();
+}
 
 // Darts for N hits: the expected count is M ln(M / (M - N)) = 2N ln 2 for
 // M = 2N, with a standard deviation below sqrt(N); eight of them plus a

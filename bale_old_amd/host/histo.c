@@ -31,16 +31,16 @@
 
 static void usage(void) {
   fprintf(stderr,
-          "Usage:\n"
-          "histo [-h][-b count][-M mask][-n num][-T tabsize][-c num][-d][-S seed]\n"
-          "- -h prints this help message\n"
-          "- -b count is the number of packages in an exstack(2) buffer\n"
-          "- -M mask is the or of 1,2,4,8,16 for the models: agi,exstack,exstack2,conveyor,alternate\n"
-          "- -n num is the number of updates per thread\n"
-          "- -T tabsize is the table size or number of buckets per thread\n"
-          "- -c num number of cores/node is a scaling factor to adjust the update rate to handle multi-core nodes\n"
-          "- -d keep the arrays in GPU memory\n"
-          "- -S seed draw indices from splitmix64 stream `seed` instead of rand()\n\n");
+          "histo: counts[index[i]] += 1 on one MI355X, for each selected model.\n"
+          "options (defaults in brackets):\n"
+          "  -n N     updates per PE [1000000]\n"
+          "  -T N     counters per PE [1000]\n"
+          "  -M MASK  models to run, a sum of 1 AGI, 2 exstack, 4 exstack2, 8 conveyor, 16 alternates [15]\n"
+          "  -b N     buffer length of the buffered models (accepted; the GPU sizes its own)\n"
+          "  -c N     cores per node: only scales the reported GB/s\n"
+          "  -d       give the library device arrays instead of host arrays\n"
+          "  -S SEED  indices from splitmix64 stream SEED instead of the reference's rand() sequence\n"
+          "  -h       this text\n");
   exit(0);
 }
 

@@ -12,16 +12,15 @@
 
 static void usage(void) {
   fprintf(stderr,
-          "This is a demo program that runs various implementations of the permute_matrix kernel.\n"
-          "Usage:\n"
-          "permute_matrix [-h][-e prob][-M mask][-n num][-s seed][-Z num]\n"
-          " -h Print usage banner only\n"
-          " -b count is the number of packages in an exstack(2) buffer\n"
-          " -e=prob Set the Erdos-Renyi probability to p.\n"
-          " -M=mask Set the models mask (1,2,4,8,16,32 for AGI, exstack, exstack2,conveyor,alternate)\n"
-          " -n=num Set the number of rows per PE to n (default = 1000).\n"
-          " -s=seed Set a seed for the random number generation.\n"
-          " -Z=num  Set the avg number of nonzeros per row to z (default = 10, overrides Erdos-Renyi p).\n\n");
+          "permute_matrix: rows and columns of an Erdos-Renyi matrix permuted on one MI355X.\n"
+          "options (defaults in brackets):\n"
+          "  -n N     rows per PE [10000]\n"
+          "  -Z N     mean nonzeros per row [10] (sets the edge probability)\n"
+          "  -e P     edge probability (used when -Z is not given)\n"
+          "  -M MASK  models to run, a sum of 1 AGI, 2 exstack, 4 exstack2, 8 conveyor [15]\n"
+          "  -s SEED  seed of the matrix and the permutations [101892]\n"
+          "  -b N     buffer length of the buffered models (accepted; the GPU sizes its own)\n"
+          "  -h       this text\n");
   exit(0);
 }
 

@@ -2,8 +2,9 @@
  * randperm.c -- the reference's randperm driver (apps/randperm_src/randperm.upc:86-175)
  * on one MI355X through libbale_hip.so: the same options, one line per model
  * with its seconds, and is_perm on every result.  At one PE the four models
- * give the permutation rand_permp_agi gives; the alternate model
- * (rand_permp_agi_opt) is not provided.
+ * give the permutation rand_permp_agi gives, and the alternate model the one
+ * rand_permp_agi_opt gives (a Fisher-Yates pass,
+ * apps/randperm_src/alternates/randperm_agi_opt.upc).
  */
 #include <getopt.h>
 #include <stdint.h>
@@ -12,13 +13,14 @@
 
 static void usage(void) {
   fprintf(stderr,
-          "Usage:\n"
-          "randperm [-h][-n num][-M mask][-s seed]\n"
-          " -h Print usage banner only\n"
-          " -b count is the number of packages in an exstack(2) buffer\n"
-          " -n num Set the permutation entries per PE to n (default = 1000).\n"
-          " -M mask Set the models mask (1,2,4,8,16,32 for AGI, exstack, exstack2,conveyor,alternate)\n"
-          " -s seed Set a seed for the random number generation.\n\n");
+          "randperm: a random permutation of 0..N-1 on one MI355X, by dart throwing\n"
+          "(models 1-8) or by a Fisher-Yates pass (model 16).\n"
+          "options (defaults in brackets):\n"
+          "  -n N     entries per PE [1000000]\n"
+          "  -M MASK  models to run, a sum of 1 AGI, 2 exstack, 4 exstack2, 8 conveyor, 16 alternate [15]\n"
+          "  -s SEED  seed of the drand48 stream [101892]\n"
+          "  -b N     buffer length of the buffered models (accepted; the GPU sizes its own)\n"
+          "  -h       this text\n");
   exit(0);
 }
 
@@ -67,8 +69,9 @@ int main(int argc, char *argv[]) {
         fprintf(stderr, "rand_permp_CONVEYOR:      ");
         break;
       case ALTERNATE_Model:
-        fprintf(stderr, "There is no alternate model here!\n");
-        continue;
+        out = rand_permp_agi_opt(numrows, (int)seed);
+        fprintf(stderr, "rand_permp_agi_opt :      ");
+        break;
       default:
         continue;
     }

@@ -3,27 +3,114 @@
  * on one MI355X through libbale_hip.so: the input from generate_toposort_input
  * (an upper-triangular Erdos-Renyi matrix with a full diagonal, rows and
  * columns permuted with rand_permp seeds 1230 and 45), its transpose, each
- * model's seconds, and check_is_triangle on every result.  Reading a Matrix
- * Market file (-f) is not provided.
+ * model's seconds, and check_is_triangle on every result.  With -f the
+ * input is a Matrix Market file instead, read as the reference's
+ * read_matrix_mm_to_dist does (spmat/spmat_utils.upc:217-365).
  */
 #include <getopt.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "driver.h"
 
 static void usage(void) {
   fprintf(stderr,
-          "Usage:\n"
-          "topo [-h][-b count][-M mask][-n num][-f filename][-Z num][-e prob][-D]\n"
-          " -h prints this help message\n"
-          " -b count is the number of packages in an exstack(2) buffer\n"
-          " -M mask is the or of 1,2,4,8,16 for the models: agi,exstack,exstack2,conveyor,alternate\n"
-          " -n num is the number of rows per thread\n"
-          " -f filename read the input matrix from filename (in Matrix Market format)\n"
-          " -Z num use an Erdos Renyi matrix with num being the expected number of nonzeros in a row \n"
-          " -e prob use an Erdos Renyi matrix where prob is the probability of an entry in matrix being non-zero \n"
-          " -D debugging flag that dumps out input and output files.\n\n");
+          "topo: the row and column permutations that make a permuted upper-triangular matrix\n"
+          "      upper triangular again, found on one MI355X.\n"
+          "options (defaults in brackets):\n"
+          "  -n N     rows per PE of the generated matrix [1000]\n"
+          "  -Z N     mean nonzeros per row of the generated matrix [35] (sets the edge probability)\n"
+          "  -e P     edge probability of the generated matrix (used when -Z is not given)\n"
+          "  -f FILE  read the matrix from FILE (Matrix Market coordinate format) instead\n"
+          "  -M MASK  models to run, a sum of 1 AGI, 2 exstack, 4 exstack2, 8 conveyor, 16 alternates [15]\n"
+          "  -b N     buffer length of the buffered models (accepted; the GPU sizes its own)\n"
+          "  -D       accepted, unused (as in the reference)\n"
+          "  -h       this text\n");
   exit(0);
+}
+
+/* A Matrix Market coordinate file ("pattern" or "integer" entries, 1-based
+ * row and column, values ignored) as a matrix in device memory, each row's
+ * columns in file order, as spmat/spmat_utils.upc:217-365 builds it.  NULL
+ * (with a message) on a bad file. */
+static sparsemat_t *read_matrix_mm(const char *name) {
+  FILE *fp = fopen(name, "r");
+  if (!fp) {
+    fprintf(stderr, "ERROR: read_matrix_mm: cannot open %s\n", name);
+    return NULL;
+  }
+  char object[64] = "", format[64] = "", field[64] = "";
+  if (fscanf(fp, "%%%%MatrixMarket %63s %63s %63s", object, format, field) != 3) {
+    fprintf(stderr, "ERROR: read_matrix_mm: bad read of header!\n");
+    fclose(fp);
+    return NULL;
+  }
+  if (strcmp(object, "matrix") || strcmp(format, "coordinate") ||
+      (strcmp(field, "integer") && strcmp(field, "pattern"))) {
+    fprintf(stderr, "ERROR: read_matrix_mm: unsupported matrix market format %s %s %s\n", object,
+            format, field);
+    fclose(fp);
+    return NULL;
+  }
+  /* skip the rest of the banner line and comment lines */
+  int c;
+  while ((c = fgetc(fp)) != EOF && c != '\n') {}
+  while ((c = fgetc(fp)) == '%')
+    while ((c = fgetc(fp)) != EOF && c != '\n') {}
+  if (c != EOF) ungetc(c, fp);
+  int64_t nr = 0, nc = 0, nnz = 0;
+  if (fscanf(fp, "%ld %ld %ld", &nr, &nc, &nnz) != 3 || nr <= 0 || nc <= 0 || nnz < 0) {
+    fprintf(stderr, "ERROR: read_matrix_mm: bad size line\n");
+    fclose(fp);
+    return NULL;
+  }
+  int64_t *Is = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
+  int64_t *Js = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
+  int64_t *off = calloc((size_t)nr + 1, sizeof(int64_t));
+  int64_t *col = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
+  if (!Is || !Js || !off || !col) {
+    fprintf(stderr, "ERROR: read_matrix_mm: could not allocate arrays\n");
+    fclose(fp);
+    free(Is); free(Js); free(off); free(col);
+    return NULL;
+  }
+  const int pattern = !strcmp(field, "pattern");
+  int64_t pos = 0, row, cl, val;
+  while (pos <= nnz) {
+    int got = pattern ? fscanf(fp, "%ld %ld", &row, &cl) : fscanf(fp, "%ld %ld %ld", &row, &cl, &val);
+    if (got == EOF) break;
+    if (got != (pattern ? 2 : 3) || row < 1 || row > nr || cl < 1 || cl > nc || pos == nnz) {
+      pos = -1;
+      break;
+    }
+    Is[pos] = row - 1;
+    Js[pos++] = cl - 1;
+    off[row]++;
+  }
+  fclose(fp);
+  sparsemat_t *A = NULL;
+  if (pos != nnz) {
+    fprintf(stderr, "ERROR: read_matrix_mm: %s entries (expected %ld)\n",
+            pos < 0 ? "malformed or out-of-range" : "wrong number of", nnz);
+  } else {
+    for (int64_t i = 0; i < nr; i++) off[i + 1] += off[i];
+    int64_t *next = malloc(sizeof(int64_t) * (size_t)nr);
+    if (next) {
+      memcpy(next, off, sizeof(int64_t) * (size_t)nr);
+      for (int64_t k = 0; k < nnz; k++) col[next[Is[k]]++] = Js[k];
+      free(next);
+      A = init_matrix(nr, nc, nnz);
+      if (A && (bale_memput(A->offset, off, (nr + 1) * 8) != BALE_OK ||
+                (nnz && bale_memput(A->nonzero, col, nnz * 8) != BALE_OK))) {
+        clear_matrix(A);
+        free(A);
+        A = NULL;
+      }
+      if (!A) fprintf(stderr, "ERROR: read_matrix_mm: %s\n", bale_hip_last_error());
+    }
+  }
+  free(Is); free(Js); free(off); free(col);
+  return A;
 }
 
 /* toposort.upc:147-167 */
@@ -79,6 +166,7 @@ int main(int argc, char *argv[]) {
   int64_t l_numrows = 100000, buf_cnt = 1024, rand_seed = 5, models_mask = ALL_Models, cores = 1;
   double nz_per_row = 35, erdos_renyi_prob = 0.0;
   int printhelp = 0, read_graph = 0, opt;
+  char filename[4096] = "";
   while ((opt = getopt(argc, argv, "hb:c:M:n:f:Z:e:D")) != -1) {
     switch (opt) {
       case 'h': printhelp = 1; break;
@@ -86,7 +174,7 @@ int main(int argc, char *argv[]) {
       case 'c': sscanf(optarg, "%ld", &cores); break;
       case 'M': sscanf(optarg, "%ld", &models_mask); break;
       case 'n': sscanf(optarg, "%ld", &l_numrows); break;
-      case 'f': read_graph = 1; break;
+      case 'f': read_graph = 1; snprintf(filename, sizeof filename, "%s", optarg); break;
       case 'Z': sscanf(optarg, "%lf", &nz_per_row); break;
       case 'e': sscanf(optarg, "%lf", &erdos_renyi_prob); break;
       case 'D': break;
@@ -94,11 +182,18 @@ int main(int argc, char *argv[]) {
     }
   }
   if (printhelp) usage();
-  if (read_graph) {
-    fprintf(stderr, "ERROR: reading a Matrix Market file (-f) is not provided\n");
-    return 1;
-  }
   driver_start(argc, argv);
+  sparsemat_t *mat = NULL;
+  if (read_graph) {
+    mat = read_matrix_mm(filename);
+    if (!mat) exit(1);
+    l_numrows = mat->numrows;
+    if (mat->numcols != mat->numrows) {
+      fprintf(stderr, "ERROR: %s is %ld x %ld, toposort needs a square matrix\n", filename,
+              mat->numrows, mat->numcols);
+      exit(1);
+    }
+  }
   const int64_t numrows = l_numrows, numcols = numrows;
   if (erdos_renyi_prob == 0.0) { /* toposort.upc:257-263 */
     erdos_renyi_prob = (2.0 * nz_per_row) / (numrows - 1);
@@ -112,7 +207,7 @@ int main(int argc, char *argv[]) {
   fprintf(stderr, "Avg # of nonzeros per row      (-Z)   %2.2lf\n", nz_per_row);
   fprintf(stderr, "Erdos-Renyi edge probability   (-e)   %lf\n", erdos_renyi_prob);
   fprintf(stderr, "task mask (M) = %ld (should be 1,2,4,8,16 for agi, exstack, exstack2, conveyors, alternates\n", models_mask);
-  sparsemat_t *mat = generate_toposort_input(numrows, erdos_renyi_prob, rand_seed);
+  if (!mat) mat = generate_toposort_input(numrows, erdos_renyi_prob, rand_seed);
   if (!mat) {
     printf("ERROR: mat is NULL!\n");
     exit(1);

@@ -13,16 +13,17 @@
 
 static void usage(void) {
   fprintf(stderr,
-          "This is a demo program that runs various implementations of the transpose_matrix kernel.\n"
-          "Usage:\n"
-          "histo [-h][-b count][-M mask][-n num][-T tabsize][-c num]\n"
-          "-h prints this help message\n"
-          "-b count is the number of packages in an exstack(2) buffer\n"
-          "-e=p Set the Erdos-Renyi probability to p.\n"
-          "-M=m Set the models mask (1,2,4,8,16,32 for AGI, exstack, exstack2,conveyor,alternate)\n"
-          "-n=n Set the number of rows per PE to n (default = 1000).\n"
-          "-s=s Set a seed for the random number generation.\n"
-          "-Z=z Set the avg number of nonzeros per row to z (default = 10, overrides Erdos-Renyi p).\n\n");
+          "transpose_matrix: the transpose of an Erdos-Renyi matrix on one MI355X.\n"
+          "options (defaults in brackets):\n"
+          "  -n N     rows per PE [10000]\n"
+          "  -Z N     mean nonzeros per row [10] (sets the edge probability)\n"
+          "  -e P     edge probability (used when -Z is not given)\n"
+          "  -M MASK  models to run, a sum of 1 AGI, 2 exstack, 4 exstack2, 8 conveyor [15]\n"
+          "  -s SEED  seed of the matrix [101892]\n"
+          "  -b N     buffer length of the buffered models (accepted; the GPU sizes its own)\n"
+          "  -c N     cores per node (accepted, unused)\n"
+          "  -C       skip the check that transposing twice gives the input back\n"
+          "  -h       this text\n");
   exit(0);
 }
 

@@ -82,6 +82,12 @@ int64_t *rand_permp_agi(int64_t N, int seed);
 int64_t *rand_permp_exstack(int64_t N, int seed, int64_t buf_cnt);
 int64_t *rand_permp_exstack2(int64_t N, int seed, int64_t buf_cnt);
 int64_t *rand_permp_conveyor(int64_t N, int seed);
+/* apps/randperm_src/randperm_alternates.h:42 -- the alternate model
+ * (apps/randperm_src/alternates/randperm_agi_opt.upc): N generator draws for
+ * the PE choice, then a Fisher-Yates pass over 0..N-1, resolved in parallel;
+ * the reference's result at one PE for the same seed.  Device memory from
+ * bale_all_alloc, NULL on error; N < 2^30. */
+int64_t *rand_permp_agi_opt(int64_t N, int seed);
 /* spmat/spmat.h:152 -- 1 if perm (device or host memory) holds 0..N-1 exactly once. */
 int is_perm(int64_t *perm, int64_t N);
 /* spmat/spmat_utils.upc:92-107 -- the reference's wrapper (its default model, AGI). */
@@ -275,6 +281,10 @@ int bale_gather_plan(int64_t n, int64_t table_len, bale_config *out);
  * at one PE for `seed` (seed 0: drand48's initial state, i.e. no srand48).
  * n < 2^30.  Waits for `stream` once, to check that the darts sufficed. */
 int bale_randperm(int64_t *perm, int64_t n, int64_t seed, void *stream);
+/* perm[0..n) (device memory) = the permutation rand_permp_agi_opt gives at
+ * one PE for `seed` (seed 0: drand48's initial state).  n < 2^30.  Only
+ * enqueues on `stream`. */
+int bale_randperm_shuffle(int64_t *perm, int64_t n, int64_t seed, void *stream);
 /* 1 if perm[0..n) (device memory) holds 0..n-1 exactly once, 0 if not,
  * negative BALE_ERR_* on error; waits for `stream`. */
 int bale_is_perm(const int64_t *perm, int64_t n, void *stream);
@@ -389,10 +399,6 @@ int bale_probe_copy(void *dst, const void *src, int64_t bytes, void *stream);
  * read number (no index stream); a digest goes to *out (one int64). */
 int bale_probe_random_read(const int64_t *table, int64_t n, int64_t nreads,
                            int64_t *out, void *stream);
-/* The same through buffer loads with cache policy `policy` (bit 0 sc0, bit 1
- * nt, bit 4 sc1; one of 0,1,2,3,16,17,18,19); table at most 4 GiB. */
-int bale_probe_random_read_policy(const int64_t *table, int64_t n, int64_t nreads,
-                                  int64_t *out, int policy, void *stream);
 
 /* ---- exstack and exstack2 ---------------------------------------------
  * The aggregation buffers of exstack/exstack.h:94-107 (bulk synchronous) and

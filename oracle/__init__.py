@@ -64,6 +64,7 @@ def lib():
             "or_lrand48_seq": (None, [U, I, P]),
             "or_glibc_lrand48": (None, [I, I, P]),
             "or_randperm_agi": (I, [P, I, I]),
+            "or_randperm_agi_opt": (None, [P, I, I]),
             "or_is_perm": (C.c_int, [P, I]),
             "or_det_log": (C.c_double, [C.c_double]),
             "or_gen_er": (I, [I, C.c_double, C.c_int, C.c_int, U, P, P]),
@@ -235,6 +236,13 @@ def randperm_agi(N: int, seed: int) -> tuple[np.ndarray, int]:
     darts = lib().or_randperm_agi(_p(perm), N, seed)
     assert darts >= 0
     return perm[:N], darts
+
+
+def randperm_agi_opt(N: int, seed: int) -> np.ndarray:
+    """rand_permp_agi_opt at one PE: Fisher-Yates after N generator draws."""
+    perm = np.empty(max(N, 1), np.int64)
+    lib().or_randperm_agi_opt(_p(perm), N, seed)
+    return perm[:N]
 
 
 def is_perm(perm: np.ndarray, N: int) -> bool:

@@ -551,6 +551,26 @@ int64_t or_randperm_agi(int64_t *perm, int64_t N, int64_t seed) {
   return darts;
 }
 
+/* rand_permp_agi_opt at one PE (apps/randperm_src/alternates/randperm_agi_opt.upc):
+ * every item draws its PE as lrand48() % THREADS (:38-42), N draws that leave
+ * nothing to choose at one PE but advance the generator; the items reach PE 0
+ * in order (:63-76), so tmp_perm = 0..N-1; then a Fisher-Yates pass
+ * j = i + lrand48() % (N - i), swap tmp[i] and tmp[j] (:92-97), and the
+ * result is tmp (:99-103).  srand48(seed) only when seed != 0 (:25). */
+void or_randperm_agi_opt(int64_t *perm, int64_t N, int64_t seed) {
+  if (seed != 0) srand48((long)(int)seed);
+  for (int64_t i = 0; i < N; i++) {
+    (void)(lrand48() % 1);
+    perm[i] = i;
+  }
+  for (int64_t i = 0; i < N; i++) {
+    const int64_t j = i + lrand48() % (N - i);
+    const int64_t t = perm[j];
+    perm[j] = perm[i];
+    perm[i] = t;
+  }
+}
+
 /* ---- sparse matrices ------------------------------------------------------ */
 
 /* log(x), x in (0, 1]: x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m =

@@ -134,6 +134,9 @@ void or_glibc_lrand48(int64_t seed, int64_t n, int64_t *out);
 /* rand_permp_agi at one PE (spmat/spmat_agi.upc:56-107) into perm[0..N);
  * returns the darts thrown (-1: out of memory). */
 int64_t or_randperm_agi(int64_t *perm, int64_t N, int64_t seed);
+/* rand_permp_agi_opt at one PE (apps/randperm_src/alternates/randperm_agi_opt.upc):
+ * N draws for the PE choice, then Fisher-Yates over 0..N-1 into perm[0..N). */
+void or_randperm_agi_opt(int64_t *perm, int64_t N, int64_t seed);
 /* is_perm (spmat/spmat_utils.upc:508): 1 if perm holds 0..N-1 once each. */
 int or_is_perm(const int64_t *perm, int64_t N);
 

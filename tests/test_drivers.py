@@ -4,6 +4,7 @@ import re
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 BUILD = Path(__file__).resolve().parent.parent / "bale_old_amd" / "_build"
@@ -40,7 +41,7 @@ def test_ig_driver_all_models(extra):
 def test_histo_driver_help():
     r = run("histo", "-h")
     assert r.returncode == 0
-    assert "Usage:" in r.stderr and "-M mask" in r.stderr
+    assert "options (defaults in brackets)" in r.stderr and "-M MASK" in r.stderr
 
 
 def test_randperm_driver_all_models():
@@ -49,7 +50,7 @@ def test_randperm_driver_all_models():
     assert "Running randperm on 1 threads" in r.stderr
     for m in ("AGI", "EXSTACK", "EXSTACK2", "CONVEYOR"):
         assert re.search(rf"rand_permp_{m}:\s+[\d.]+", r.stderr), m
-    assert "There is no alternate model here!" in r.stderr
+    assert re.search(r"rand_permp_agi_opt :\s+[\d.]+", r.stderr)  # the alternate model
     assert "BALE FAIL" not in r.stderr and "ERROR" not in r.stdout
 
 
@@ -70,10 +71,66 @@ def test_permute_driver_all_models():
 
 
 def test_toposort_driver_all_models():
-    r = run("toposort", "-n", "20000", "-M", "31")
+    r = run("topo", "-n", "20000", "-M", "31")
     assert r.returncode == 0, r.stderr + r.stdout
     assert "Running toposort on 1 threads" in r.stderr
     assert re.search(r"Input matrix has 20000 rows and \d+ nonzeros", r.stdout)
     assert len(re.findall(r"\s+[\d.]+ seconds", r.stderr)) == 5
     assert "ERROR" not in r.stderr and "ERROR" not in r.stdout
 
+
+
+def write_mm(path, n, rows, cols, field="pattern", nnz=None, banner=None):
+    lines = [banner or f"%%MatrixMarket matrix coordinate {field} general", "% written by the test",
+             f"{n} {n} {len(rows) if nnz is None else nnz}"]
+    for r, c in zip(rows, cols):
+        lines.append(f"{r + 1} {c + 1}" + (" 1" if field == "integer" else ""))
+    path.write_text("\n".join(lines) + "\n")
+
+
+def permuted_upper_triangle(n, seed):
+    """A unit upper-triangular pattern under random row and column
+    permutations, entries in random order: what toposort undoes."""
+    rng = np.random.default_rng(seed)
+    r, c = np.triu_indices(n)
+    keep = (r == c) | (rng.random(r.size) < 0.05)
+    r, c = r[keep], c[keep]
+    rp, cp = rng.permutation(n), rng.permutation(n)
+    order = rng.permutation(r.size)
+    return rp[r][order], cp[c][order]
+
+
+@pytest.mark.parametrize("field", ["pattern", "integer"])
+def test_topo_driver_reads_matrix_market(tmp_path, field):
+    """-f: the matrix comes from a Matrix Market file
+    (apps/topo_src/toposort.upc:252, spmat/spmat_utils.upc:217-365) and every
+    model toposorts it."""
+    n = 700
+    rows, cols = permuted_upper_triangle(n, seed=11)
+    f = tmp_path / "m.mtx"
+    write_mm(f, n, rows, cols, field)
+    r = run("topo", "-f", str(f), "-M", "31")
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert f"Input matrix has {n} rows and {rows.size} nonzeros" in r.stdout
+    assert len(re.findall(r"\s+[\d.]+ seconds", r.stderr)) == 5
+    assert "ERROR" not in r.stderr and "ERROR" not in r.stdout
+
+
+@pytest.mark.parametrize("case", ["banner", "array", "count", "range"])
+def test_topo_driver_rejects_bad_matrix_market(tmp_path, case):
+    n = 50
+    rows, cols = permuted_upper_triangle(n, seed=3)
+    f = tmp_path / "bad.mtx"
+    if case == "banner":
+        f.write_text("not a matrix market file\n")
+    elif case == "array":
+        write_mm(f, n, rows, cols, banner="%%MatrixMarket matrix array real general")
+    elif case == "count":
+        write_mm(f, n, rows, cols, nnz=rows.size + 1)
+    else:
+        rows = rows.copy()
+        rows[7] = n  # written 1-based as n + 1
+        write_mm(f, n, rows, cols)
+    r = run("topo", "-f", str(f))
+    assert r.returncode != 0
+    assert "ERROR: read_matrix_mm" in r.stderr
