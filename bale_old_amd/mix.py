@@ -164,12 +164,22 @@ def launch_each(inst, count: int, stream=None) -> None:
     check(lib().bale_batch_launch_each(inst, count, _stream(stream)))
 
 
-def tune(inst, count: int, min_ops: int = 1 << 22, max_cand: int = 32, stream=None):
+def tune(inst, count: int, min_ops: int = 1 << 22, max_cand: int = 32, stream=None,
+         chunk: int = 0, log=None):
     """Exhaustive autotuner: fills inst[i].cfg; returns the cost table
-    (count x max_cand seconds per op, -1 where unused) as a list of lists."""
+    (count x max_cand seconds per op, -1 where unused) as a list of lists.
+    Instances are tuned independently; with `chunk` they go to the library
+    `chunk` at a time and `log` (if given) hears after each."""
     from . import _stream
     costs = (C.c_float * (count * max_cand))()
-    check(lib().bale_tune(inst, count, min_ops, costs, max_cand, _stream(stream)))
+    step = chunk if chunk > 0 else max(count, 1)
+    for k0 in range(0, count, step):
+        k = min(step, count - k0)
+        part = C.cast(C.addressof(inst) + k0 * C.sizeof(Instance), C.POINTER(Instance))
+        cpart = C.cast(C.addressof(costs) + k0 * max_cand * C.sizeof(C.c_float), C.POINTER(C.c_float))
+        check(lib().bale_tune(part, k, min_ops, cpart, max_cand, _stream(stream)))
+        if log is not None:
+            log(f"tuned {k0 + k}/{count} instances")
     return [list(costs[i * max_cand:(i + 1) * max_cand]) for i in range(count)]
 
 

@@ -165,9 +165,10 @@ def bench_batch(B, torch, dev, stream, seed, count, reps, min_ops, learned):
     B.sync_errors(stream)
     out["plain_ms"] = t_plain * 1e3
     out["plain_GUPS"] = ops / t_plain / 1e9
+    log(f"seed {seed}: plain launches {t_plain * 1e3:.1f} ms; tuning ...")
     mix.reset_counts()
     t0 = time.time()
-    MX.tune(mix.inst, count, min_ops=min_ops, stream=stream)
+    MX.tune(mix.inst, count, min_ops=min_ops, stream=stream, chunk=512, log=lambda m: log(f"seed {seed}: {m}"))
     torch.cuda.synchronize()
     out["tune_seconds"] = time.time() - t0
     tuned = mix.configs()
@@ -189,6 +190,7 @@ def bench_batch(B, torch, dev, stream, seed, count, reps, min_ops, learned):
     # one library call per instance; the library's default rule and DIRECT
     # everywhere, each in one graph like the tuned batch
     mix.reset_counts()
+    log(f"seed {seed}: exhaustive batch {t_ex * 1e3:.2f} ms")
     t_each = time_calls(lambda: MX.launch_each(mix.inst, count, stream), reps, stream)
     B.sync_errors(stream)
     check_mix_sample(mix, reps + 1)
@@ -210,7 +212,8 @@ def bench_batch(B, torch, dev, stream, seed, count, reps, min_ops, learned):
     # labels' own noise, an upper bound for any selector's agreement)
     sub = list(range(0, count, max(1, count // 256)))
     sub_inst = (MX.Instance * len(sub))(*[mix.inst[i] for i in sub])
-    MX.tune(sub_inst, len(sub), min_ops=min_ops, stream=stream)
+    log(f"seed {seed}: re-tuning {len(sub)} instances ...")
+    MX.tune(sub_inst, len(sub), min_ops=min_ops, stream=stream, chunk=128, log=lambda m: log(f"seed {seed}: {m}"))
     torch.cuda.synchronize()
     again = [MX.BaleConfigCopy(sub_inst[k].cfg) for k in range(len(sub))]
     out["retune_agreement"] = {"instances": len(sub),
