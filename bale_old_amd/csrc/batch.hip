@@ -570,8 +570,16 @@ int plan_add(Plan& P, const bale_instance& in, const Device* d) {
     I.C = (int32_t)C;
     I.chunk = task;
     I.flags = (c.hot ? kFlagHot : 0) | (c.merge > 0 ? kFlagRuns : 0);
-    int S = std::min(64, part_pick_depth((int)B, sizeof(uint16_t), (size_t)d->lds_per_block, c.hot != 0));
+    const size_t budget = (size_t)d->lds_per_block;
+    int S = std::min(64, part_pick_depth((int)B, sizeof(uint16_t), budget, c.hot != 0));
     if (S < 4) return BALE_ERR_ARG;
+    // Few buckets: deeper buffers (up to 8192 entries, 64 KiB of entries in
+    // all).  A round places 8 entries per thread, so with one to four buckets
+    // a 64-entry buffer overflows dozens of times per round, each overflow a
+    // barrier and a ship.
+    while (S < 8192 && B * (int64_t)S * 2 * (int64_t)sizeof(uint16_t) <= ((int64_t)64 << 10) &&
+           part_scatter_lds((int)B, 2 * S, sizeof(uint16_t), c.hot != 0) <= budget)
+      S *= 2;
     I.S = S;
     P.lds_c = std::max(P.lds_c, part_scatter_lds((int)B, S, sizeof(uint16_t), c.hot != 0));
     int64_t seg = std::max<int64_t>({(int64_t)4 << tb, 16384, ceil_div(in.n, 64)});
