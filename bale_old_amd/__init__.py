@@ -101,6 +101,7 @@ def _proto(L) -> None:
     f("bale_gather", C.c_int, VP, VP, I64T, VP, I64T, C.c_int, C.POINTER(BaleConfig), VP)
     f("bale_sync_errors", C.c_int, VP, P64)
     f("bale_hip_release_stream", C.c_int, VP)
+    f("bale_hip_set_alc8r", C.c_int, VP)
     f("bale_histo_noalloc", C.c_int, VP, I64T, VP, I64T, C.c_int, C.c_int, C.POINTER(BaleConfig), VP, VP)
     f("bale_hip_last_strategy", C.c_int)
     f("bale_histo_plan", C.c_int, I64T, I64T, C.c_int, C.POINTER(BaleConfig))

@@ -112,6 +112,13 @@ int bale_set_error(int code, const char* fmt, ...);
 // Workspace of at least `bytes` for kernels on s, 256-byte aligned; nullptr on
 // failure.
 void* workspace(Device* d, size_t bytes, hipStream_t s);
+// Device memory for the library's own use: through the allocator installed
+// with bale_hip_set_alc8r (hipMalloc by default), tagged with the calling
+// file and line; nullptr on failure (no error text is set).  dev_free returns
+// a pointer to the allocator that made it (nullptr is ignored).
+void* dev_alloc(size_t bytes, const char* tag, uint64_t line);
+void dev_free(void* p);
+#define BALE_DEV_ALLOC(bytes) ::bale::dev_alloc((bytes), __FILE_NAME__, __LINE__)
 // BALE_ERR_HIP with the runtime's text for rc (bale_hip_last_error).
 int bale_fail_hip(int rc, const char* what);
 // What a launcher returns for a configuration that cannot run on this shape

@@ -464,18 +464,13 @@ namespace {
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
+  ~DevBuf() { dev_free(p); }
   int ensure(size_t bytes) {
     if (cap >= bytes) return 0;
-    if (p) (void)hipFree(p);
+    dev_free(p);
     p = nullptr;
     cap = 0;
-    if (hipMalloc(&p, bytes ? bytes : 256) != hipSuccess) {
-      (void)hipGetLastError();
-      return (int)hipErrorOutOfMemory;
-    }
+    if (!(p = BALE_DEV_ALLOC(bytes ? bytes : 256))) return (int)hipErrorOutOfMemory;
     cap = bytes ? bytes : 256;
     return 0;
   }
@@ -671,16 +666,23 @@ int plan_upload(Plan& P, Buffers& Bf, hipStream_t s) {
     if (bytes) return (int)hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
     return 0;
   };
-  if ((rc = up(Bf.inst, P.inst.data(), P.inst.size() * sizeof(BInst)))) return rc;
-  if ((rc = up(Bf.a, P.a.data(), P.a.size() * sizeof(BTask)))) return rc;
-  if ((rc = up(Bf.qoff, P.qoff.data(), P.qoff.size() * 4))) return rc;
-  if ((rc = up(Bf.c, P.c.data(), P.c.size() * sizeof(BTask)))) return rc;
-  if ((rc = up(Bf.d, P.d.data(), P.d.size() * sizeof(BTask)))) return rc;
-  if ((rc = up(Bf.hot_list, P.hot_list.data(), P.hot_list.size() * 4))) return rc;
-  if ((rc = up(Bf.bucket_list, P.bucket_list.data(), P.bucket_list.size() * 4))) return rc;
-  if ((rc = Bf.heads.ensure(64))) return rc;
-  if ((rc = Bf.err.ensure(sizeof(ErrWords)))) return rc;
-  if ((rc = (int)hipMemsetAsync(Bf.err.p, 0, sizeof(ErrWords), s))) return rc;
+  // on failure, copies already queued finish before the caller frees their
+  // buffers and sources
+  auto failed = [&](int r) {
+    (void)hipStreamSynchronize(s);
+    (void)hipGetLastError();
+    return r;
+  };
+  if ((rc = up(Bf.inst, P.inst.data(), P.inst.size() * sizeof(BInst)))) return failed(rc);
+  if ((rc = up(Bf.a, P.a.data(), P.a.size() * sizeof(BTask)))) return failed(rc);
+  if ((rc = up(Bf.qoff, P.qoff.data(), P.qoff.size() * 4))) return failed(rc);
+  if ((rc = up(Bf.c, P.c.data(), P.c.size() * sizeof(BTask)))) return failed(rc);
+  if ((rc = up(Bf.d, P.d.data(), P.d.size() * sizeof(BTask)))) return failed(rc);
+  if ((rc = up(Bf.hot_list, P.hot_list.data(), P.hot_list.size() * 4))) return failed(rc);
+  if ((rc = up(Bf.bucket_list, P.bucket_list.data(), P.bucket_list.size() * 4))) return failed(rc);
+  if ((rc = Bf.heads.ensure(64))) return failed(rc);
+  if ((rc = Bf.err.ensure(sizeof(ErrWords)))) return failed(rc);
+  if ((rc = (int)hipMemsetAsync(Bf.err.p, 0, sizeof(ErrWords), s))) return failed(rc);
   return (int)hipStreamSynchronize(s);
 }
 
@@ -927,6 +929,8 @@ int bale_batch_launch_plain(const bale_instance* inst, int64_t count, void* stre
 }
 
 int bale_tune_candidates(int kind, int64_t table_len, bale_config* out, int max) {
+  if (kind != 0 && kind != 1) return bale_set_error(BALE_ERR_ARG, "bad instance kind %d", kind);
+  if (table_len <= 0) return bale_set_error(BALE_ERR_ARG, "table_len must be positive");
   std::vector<bale_config> v;
   auto add = [&](int strat, int task_log2, int merge, int hot) {
     bale_config c;

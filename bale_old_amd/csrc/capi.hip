@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "bale_internal.h"
 
@@ -61,6 +62,63 @@ void set_last_timing(double gpu_s, double h2d_s, double d2h_s) {
   t_d2h_s = d2h_s;
 }
 double last_gpu_seconds() { return t_gpu_s; }
+
+// ---- device allocations ---------------------------------------------------
+// The installed allocator (a null grab means hipMalloc / hipFree), and which
+// allocator made each pointer that a caller's allocator handed out, so that
+// every pointer goes back to the allocator that made it even after another
+// one is installed.
+struct Alc8r {
+  void* ctx = nullptr;
+  void* (*grab)(void*, size_t, const char*, uint64_t) = nullptr;
+  void (*release)(void*, void*) = nullptr;
+};
+static Alc8r g_alc8r;
+static std::mutex g_alc8r_mu;
+static std::unordered_map<void*, Alc8r>* g_ledger = nullptr;  // never destroyed
+
+void* dev_alloc(size_t bytes, const char* tag, uint64_t line) {
+  Alc8r a;
+  {
+    std::lock_guard<std::mutex> lk(g_alc8r_mu);
+    a = g_alc8r;
+  }
+  if (!bytes) bytes = 256;
+  void* p = nullptr;
+  if (!a.grab) {
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+    }
+    return p;
+  }
+  p = a.grab(a.ctx, bytes, tag, line);
+  if (p) {
+    std::lock_guard<std::mutex> lk(g_alc8r_mu);
+    if (!g_ledger) g_ledger = new std::unordered_map<void*, Alc8r>();
+    (*g_ledger)[p] = a;
+  }
+  return p;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  Alc8r a;
+  bool theirs = false;
+  {
+    std::lock_guard<std::mutex> lk(g_alc8r_mu);
+    if (g_ledger) {
+      auto it = g_ledger->find(p);
+      if (it != g_ledger->end()) {
+        a = it->second;
+        g_ledger->erase(it);
+        theirs = true;
+      }
+    }
+  }
+  if (theirs) a.release(a.ctx, p);
+  else (void)hipFree(p);
+}
 
 constexpr int kMaxDevices = 64;
 static Device g_dev[kMaxDevices];
@@ -142,15 +200,14 @@ StreamCtx* stream_ctx(Device* d, hipStream_t s) {
                        "outside capture first (or use bale_histo_noalloc)");
     return nullptr;
   }
-  ErrWords* err = nullptr;
-  if (hipMalloc((void**)&err, sizeof(ErrWords)) != hipSuccess) {
-    (void)hipGetLastError();
+  ErrWords* err = (ErrWords*)BALE_DEV_ALLOC(sizeof(ErrWords));
+  if (!err) {
     fail(BALE_ERR_ALLOC, "error record allocation failed");
     return nullptr;
   }
   if (hipMemsetAsync(err, 0, sizeof(ErrWords), s) != hipSuccess) {
     (void)hipGetLastError();
-    (void)hipFree(err);
+    dev_free(err);
     fail(BALE_ERR_HIP, "clearing the error record failed");
     return nullptr;
   }
@@ -218,14 +275,13 @@ void* workspace(Device* d, size_t bytes, hipStream_t s) {
   if (c->ws) {
     // only this stream's kernels use it
     if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
-    (void)hipFree(c->ws);
+    dev_free(c->ws);
     c->ws = nullptr;
     c->ws_bytes = 0;
   }
   size_t want = bytes + (bytes >> 3);  // grow with headroom
-  if (hipMalloc(&c->ws, want) != hipSuccess) {
-    (void)hipGetLastError();
-    c->ws = nullptr;
+  c->ws = BALE_DEV_ALLOC(want);
+  if (!c->ws) {
     fail(BALE_ERR_ALLOC, "workspace of %zu bytes could not be allocated", want);
     return nullptr;
   }
@@ -510,6 +566,13 @@ int bale_sync_errors(void* stream, int64_t* bad) {
   return BALE_OK;
 }
 
+int bale_hip_set_alc8r(const bale_alc8r_t* a) {
+  if (a && (!a->grab || !a->free)) return fail(BALE_ERR_ARG, "an allocator needs both grab and free");
+  std::lock_guard<std::mutex> lk(g_alc8r_mu);
+  g_alc8r = a ? Alc8r{a->alc8r, a->grab, a->free} : Alc8r();
+  return BALE_OK;
+}
+
 int bale_hip_release_stream(void* stream) {
   Device* d = device_state();
   if (!d) return fail(BALE_ERR_NODEV, "no usable HIP device");
@@ -519,8 +582,8 @@ int bale_hip_release_stream(void* stream) {
     if (!c.used || c.s != s) continue;
     int rc = (int)hipStreamSynchronize(s);
     if (rc) return fail_hip(rc, "hipStreamSynchronize");
-    if (c.ws) (void)hipFree(c.ws);
-    if (c.err) (void)hipFree(c.err);
+    dev_free(c.ws);
+    dev_free(c.err);
     c = StreamCtx();
   }
   return BALE_OK;
@@ -536,23 +599,21 @@ void* bale_all_alloc(int64_t nblocks, int64_t blocksize) {
     return nullptr;
   }
   size_t bytes = (size_t)(nblocks ? nblocks : 1) * (size_t)blocksize;
-  void* p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    fail(BALE_ERR_ALLOC, "hipMalloc of %zu bytes failed", bytes);
+  void* p = BALE_DEV_ALLOC(bytes);
+  if (!p) {
+    fail(BALE_ERR_ALLOC, "device allocation of %zu bytes failed", bytes);
     return nullptr;
   }
   if (hipMemset(p, 0, bytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
-    (void)hipFree(p);
+    (void)hipGetLastError();
+    dev_free(p);
     fail(BALE_ERR_HIP, "zeroing the allocation failed");
     return nullptr;
   }
   return p;
 }
 
-void bale_all_free(void* p) {
-  if (p) (void)hipFree(p);
-}
+void bale_all_free(void* p) { dev_free(p); }
 
 int bale_memput(void* dst_dev, const void* src_host, int64_t bytes) {
   if (bytes < 0 || (bytes && (!dst_dev || !src_host))) return fail(BALE_ERR_ARG, "bad copy");
@@ -617,7 +678,7 @@ struct Staged {
   void* p = nullptr;
   bool owned = false;
   ~Staged() {
-    if (owned && p) (void)hipFree(p);
+    if (owned) dev_free(p);
   }
 };
 
@@ -655,7 +716,7 @@ static double dropin(int kind, int64_t* out, int64_t* index, int64_t n, int64_t*
   if (is_device_ptr(index)) {
     sidx.p = index;
   } else {
-    if ((rc = (int)hipMalloc(&sidx.p, (size_t)n * 8))) { fail(BALE_ERR_ALLOC, "staging index"); cleanup(); return -1.0; }
+    if (!(sidx.p = BALE_DEV_ALLOC((size_t)n * 8))) { fail(BALE_ERR_ALLOC, "staging index"); cleanup(); return -1.0; }
     sidx.owned = true;
     if ((rc = (int)hipMemcpy(sidx.p, index, (size_t)n * 8, hipMemcpyHostToDevice))) { fail_hip(rc, "index H2D"); cleanup(); return -1.0; }
   }
@@ -664,11 +725,11 @@ static double dropin(int kind, int64_t* out, int64_t* index, int64_t n, int64_t*
   {
     int64_t* mm = nullptr;
     int64_t init[2] = {INT64_MAX, INT64_MIN}, got[2];
-    if (hipMalloc((void**)&mm, 16) != hipSuccess) { fail(BALE_ERR_ALLOC, "minmax"); cleanup(); return -1.0; }
+    if (!(mm = (int64_t*)BALE_DEV_ALLOC(16))) { fail(BALE_ERR_ALLOC, "minmax"); cleanup(); return -1.0; }
     (void)hipMemcpy(mm, init, 16, hipMemcpyHostToDevice);
     launch_minmax((const int64_t*)sidx.p, n, packed ? 1 : 0, mm, s);
     rc = (int)hipMemcpy(got, mm, 16, hipMemcpyDeviceToHost);
-    (void)hipFree(mm);
+    dev_free(mm);
     if (rc) { fail_hip(rc, "minmax"); cleanup(); return -1.0; }
     if (got[0] < 0) { fail(BALE_ERR_RANGE, "negative index %lld", (long long)got[0]); cleanup(); return -1.0; }
     tlen = got[1] + 1;
@@ -683,7 +744,7 @@ static double dropin(int kind, int64_t* out, int64_t* index, int64_t n, int64_t*
       return -1.0;
     }
   } else {
-    if (hipMalloc(&stab.p, (size_t)tlen * 8) != hipSuccess) { fail(BALE_ERR_ALLOC, "staging table"); cleanup(); return -1.0; }
+    if (!(stab.p = BALE_DEV_ALLOC((size_t)tlen * 8))) { fail(BALE_ERR_ALLOC, "staging table"); cleanup(); return -1.0; }
     stab.owned = true;
     if ((rc = (int)hipMemcpy(stab.p, table_or_counts, (size_t)tlen * 8, hipMemcpyHostToDevice))) { fail_hip(rc, "table H2D"); cleanup(); return -1.0; }
   }
@@ -691,7 +752,7 @@ static double dropin(int kind, int64_t* out, int64_t* index, int64_t n, int64_t*
   bool host_out = kind == 1 && !is_device_ptr(out);
   if (kind == 1) {
     if (host_out) {
-      if (hipMalloc(&sout.p, (size_t)n * 8) != hipSuccess) { fail(BALE_ERR_ALLOC, "staging tgt"); cleanup(); return -1.0; }
+      if (!(sout.p = BALE_DEV_ALLOC((size_t)n * 8))) { fail(BALE_ERR_ALLOC, "staging tgt"); cleanup(); return -1.0; }
       sout.owned = true;
     } else {
       sout.p = out;

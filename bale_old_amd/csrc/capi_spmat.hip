@@ -197,18 +197,15 @@ struct StagedI64 {
   const int64_t* p = nullptr;
   int64_t* owned = nullptr;
   ~StagedI64() {
-    if (owned) (void)hipFree(owned);
+    dev_free(owned);
   }
   int stage(const int64_t* src, int64_t n) {
     if (!src || n <= 0 || is_device_ptr(src)) {
       p = src;
       return 0;
     }
-    if (hipMalloc((void**)&owned, (size_t)n * 8) != hipSuccess) {
-      (void)hipGetLastError();
-      owned = nullptr;
-      return bale_set_error(BALE_ERR_ALLOC, "staging %lld values", (long long)n);
-    }
+    owned = (int64_t*)BALE_DEV_ALLOC((size_t)n * 8);
+    if (!owned) return bale_set_error(BALE_ERR_ALLOC, "staging %lld values", (long long)n);
     int rc = (int)hipMemcpy(owned, src, (size_t)n * 8, hipMemcpyHostToDevice);
     if (rc) return bale_fail_hip(rc, "staging to the device");
     p = owned;
@@ -222,18 +219,15 @@ struct StagedOut {
   int64_t* p = nullptr;
   int64_t* owned = nullptr;
   ~StagedOut() {
-    if (owned) (void)hipFree(owned);
+    dev_free(owned);
   }
   int stage(int64_t* dst, int64_t n) {
     if (n <= 0 || is_device_ptr(dst)) {
       p = dst;
       return 0;
     }
-    if (hipMalloc((void**)&owned, (size_t)n * 8) != hipSuccess) {
-      (void)hipGetLastError();
-      owned = nullptr;
-      return bale_set_error(BALE_ERR_ALLOC, "staging %lld values", (long long)n);
-    }
+    owned = (int64_t*)BALE_DEV_ALLOC((size_t)n * 8);
+    if (!owned) return bale_set_error(BALE_ERR_ALLOC, "staging %lld values", (long long)n);
     p = owned;
     return 0;
   }

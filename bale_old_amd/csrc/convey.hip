@@ -132,11 +132,13 @@ bale_convey* bale_convey_new(int64_t n_pe, int64_t capacity, size_t max_item_byt
   c->capacity = capacity;
   c->max_item_bytes = max_item_bytes;
   const size_t bytes = (size_t)capacity * max_item_bytes;
-  if (hipMalloc(&c->queue, bytes) != hipSuccess ||
-      hipMalloc((void**)&c->qpes, (size_t)capacity * sizeof(int64_t)) != hipSuccess ||
-      hipMalloc(&c->out, bytes) != hipSuccess) {
-    (void)hipFree(c->queue);
-    (void)hipFree(c->qpes);
+  c->queue = BALE_DEV_ALLOC(bytes);
+  c->qpes = (int64_t*)BALE_DEV_ALLOC((size_t)capacity * sizeof(int64_t));
+  c->out = BALE_DEV_ALLOC(bytes);
+  if (!c->queue || !c->qpes || !c->out) {
+    dev_free(c->queue);
+    dev_free(c->qpes);
+    dev_free(c->out);
     delete c;
     bale_set_error(BALE_ERR_ALLOC, "convey_new: cannot allocate %zu bytes",
                    2 * bytes + (size_t)capacity * 8);
@@ -233,9 +235,9 @@ int bale_convey_reset(bale_convey* c) {
 
 int bale_convey_free(bale_convey* c) {
   if (!c) return BALE_CONVEY_OK;
-  (void)hipFree(c->queue);
-  (void)hipFree(c->qpes);
-  (void)hipFree(c->out);
+  dev_free(c->queue);
+  dev_free(c->qpes);
+  dev_free(c->out);
   delete c;
   return BALE_CONVEY_OK;
 }

@@ -213,10 +213,9 @@ bale_exstack* ex_new(int kind, int64_t n_pe, int64_t buf_cnt, size_t pkg) {
   const size_t sz[11] = {ib, pb, pb, ib, pb, pb, ib, pb, (size_t)n_pe * n_pe * 4, pb, pb};
   bool ok = true;
   for (int k = 0; k < 11 && ok; k++)
-    if (hipMalloc(&p[k], std::max<size_t>(sz[k], 256)) != hipSuccess) ok = false;
+    if (!(p[k] = BALE_DEV_ALLOC(std::max<size_t>(sz[k], 256)))) ok = false;
   if (!ok) {
-    (void)hipGetLastError();
-    for (void* q : p) (void)hipFree(q);
+    for (void* q : p) dev_free(q);
     delete x;
     bale_set_error(BALE_ERR_ALLOC, "%s: cannot allocate the buffers", name);
     return nullptr;
@@ -347,7 +346,7 @@ void bale_exstack_clear(bale_exstack* x) {
   if (!x) return;
   void* p[11] = {x->q_items, x->q_src, x->q_dst, x->t_items, x->t_src, x->t_dst, x->out_items,
                  x->out_src, x->pair_dev, x->k_dst, x->k_src};
-  for (void* q : p) (void)hipFree(q);
+  for (void* q : p) dev_free(q);
   delete x;
 }
 

@@ -434,17 +434,20 @@ int bale_sketch(const bale_instance* inst, int64_t count, float* features, void*
       return bale_set_error(BALE_ERR_ARG, "instance %lld has no index", (long long)i);
     }
   }
-  SketchIn* d = nullptr;
-  int rc = (int)hipMalloc((void**)&d, sizeof(SketchIn) * count);
-  if (!rc) rc = (int)hipMemcpy(d, h, sizeof(SketchIn) * count, hipMemcpyHostToDevice);
+  SketchIn* d = (SketchIn*)BALE_DEV_ALLOC(sizeof(SketchIn) * count);
+  if (!d) {
+    delete[] h;
+    return bale_set_error(BALE_ERR_ALLOC, "sketch descriptors: device allocation failed");
+  }
+  int rc = (int)hipMemcpy(d, h, sizeof(SketchIn) * count, hipMemcpyHostToDevice);
   delete[] h;
   if (rc) {
-    if (d) (void)hipFree(d);
+    dev_free(d);
     return bale_set_error(BALE_ERR_HIP, "sketch descriptors: %s", hipGetErrorString((hipError_t)rc));
   }
   hipLaunchKernelGGL(k_sketch, dim3((unsigned)count), dim3(256), 0, s, d, count, features);
   rc = (int)hipStreamSynchronize(s);
-  (void)hipFree(d);
+  dev_free(d);
   return rc ? bale_set_error(BALE_ERR_HIP, "sketch: %s", hipGetErrorString((hipError_t)rc)) : BALE_OK;
 }
 

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include "driver.h"
+#include "mm_read.h"
 
 static void usage(void) {
   fprintf(stderr,
@@ -29,87 +30,20 @@ static void usage(void) {
   exit(0);
 }
 
-/* A Matrix Market coordinate file ("pattern" or "integer" entries, 1-based
- * row and column, values ignored) as a matrix in device memory, each row's
- * columns in file order, as spmat/spmat_utils.upc:217-365 builds it.  NULL
- * (with a message) on a bad file. */
+/* A Matrix Market file (mm_read.h) as a matrix in device memory; NULL (with
+ * a message) on a bad file. */
 static sparsemat_t *read_matrix_mm(const char *name) {
-  FILE *fp = fopen(name, "r");
-  if (!fp) {
-    fprintf(stderr, "ERROR: read_matrix_mm: cannot open %s\n", name);
-    return NULL;
+  mm_csr m;
+  if (mm_read(name, &m)) return NULL;
+  sparsemat_t *A = init_matrix(m.nr, m.nc, m.nnz);
+  if (A && (bale_memput(A->offset, m.offset, (m.nr + 1) * 8) != BALE_OK ||
+            (m.nnz && bale_memput(A->nonzero, m.nonzero, m.nnz * 8) != BALE_OK))) {
+    clear_matrix(A);
+    free(A);
+    A = NULL;
   }
-  char object[64] = "", format[64] = "", field[64] = "";
-  if (fscanf(fp, "%%%%MatrixMarket %63s %63s %63s", object, format, field) != 3) {
-    fprintf(stderr, "ERROR: read_matrix_mm: bad read of header!\n");
-    fclose(fp);
-    return NULL;
-  }
-  if (strcmp(object, "matrix") || strcmp(format, "coordinate") ||
-      (strcmp(field, "integer") && strcmp(field, "pattern"))) {
-    fprintf(stderr, "ERROR: read_matrix_mm: unsupported matrix market format %s %s %s\n", object,
-            format, field);
-    fclose(fp);
-    return NULL;
-  }
-  /* skip the rest of the banner line and comment lines */
-  int c;
-  while ((c = fgetc(fp)) != EOF && c != '\n') {}
-  while ((c = fgetc(fp)) == '%')
-    while ((c = fgetc(fp)) != EOF && c != '\n') {}
-  if (c != EOF) ungetc(c, fp);
-  int64_t nr = 0, nc = 0, nnz = 0;
-  if (fscanf(fp, "%ld %ld %ld", &nr, &nc, &nnz) != 3 || nr <= 0 || nc <= 0 || nnz < 0) {
-    fprintf(stderr, "ERROR: read_matrix_mm: bad size line\n");
-    fclose(fp);
-    return NULL;
-  }
-  int64_t *Is = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
-  int64_t *Js = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
-  int64_t *off = calloc((size_t)nr + 1, sizeof(int64_t));
-  int64_t *col = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
-  if (!Is || !Js || !off || !col) {
-    fprintf(stderr, "ERROR: read_matrix_mm: could not allocate arrays\n");
-    fclose(fp);
-    free(Is); free(Js); free(off); free(col);
-    return NULL;
-  }
-  const int pattern = !strcmp(field, "pattern");
-  int64_t pos = 0, row, cl, val;
-  while (pos <= nnz) {
-    int got = pattern ? fscanf(fp, "%ld %ld", &row, &cl) : fscanf(fp, "%ld %ld %ld", &row, &cl, &val);
-    if (got == EOF) break;
-    if (got != (pattern ? 2 : 3) || row < 1 || row > nr || cl < 1 || cl > nc || pos == nnz) {
-      pos = -1;
-      break;
-    }
-    Is[pos] = row - 1;
-    Js[pos++] = cl - 1;
-    off[row]++;
-  }
-  fclose(fp);
-  sparsemat_t *A = NULL;
-  if (pos != nnz) {
-    fprintf(stderr, "ERROR: read_matrix_mm: %s entries (expected %ld)\n",
-            pos < 0 ? "malformed or out-of-range" : "wrong number of", nnz);
-  } else {
-    for (int64_t i = 0; i < nr; i++) off[i + 1] += off[i];
-    int64_t *next = malloc(sizeof(int64_t) * (size_t)nr);
-    if (next) {
-      memcpy(next, off, sizeof(int64_t) * (size_t)nr);
-      for (int64_t k = 0; k < nnz; k++) col[next[Is[k]]++] = Js[k];
-      free(next);
-      A = init_matrix(nr, nc, nnz);
-      if (A && (bale_memput(A->offset, off, (nr + 1) * 8) != BALE_OK ||
-                (nnz && bale_memput(A->nonzero, col, nnz * 8) != BALE_OK))) {
-        clear_matrix(A);
-        free(A);
-        A = NULL;
-      }
-      if (!A) fprintf(stderr, "ERROR: read_matrix_mm: %s\n", bale_hip_last_error());
-    }
-  }
-  free(Is); free(Js); free(off); free(col);
+  if (!A) fprintf(stderr, "ERROR: read_matrix_mm: %s\n", bale_hip_last_error());
+  mm_free(&m);
   return A;
 }
 

@@ -270,6 +270,23 @@ int bale_sync_errors(void *stream, int64_t *bad);
 /* Wait for `stream` and free its error record and workspace (for callers that
  * create and destroy many streams; at most 256 are held per device). */
 int bale_hip_release_stream(void *stream);
+/* convey/convey_alc8r.h:53-60 -- the reference's allocator object
+ * (convey_alc8r_t), here for all the device memory the library allocates for
+ * itself from the call on: stream workspaces and error records,
+ * bale_all_alloc, staging copies, batches, conveyors, exstacks.  grab must
+ * return device memory of at least `size` bytes (256-byte aligned) or NULL;
+ * `tag` and `value` are the library's source file and line.  Each pointer is
+ * returned to the allocator that grabbed it, even after another is
+ * installed.  NULL restores hipMalloc / hipFree.  A failed grab makes the
+ * call that needed it return BALE_ERR_ALLOC (the drop-ins -1.0 or NULL).
+ * The reference passes an allocator to each conveyor; here it is one per
+ * process. */
+typedef struct bale_alc8r {
+  void *alc8r;
+  void *(*grab)(void *alc8r, size_t size, const char *tag, uint64_t value);
+  void (*free)(void *alc8r, void *ptr);
+} bale_alc8r_t;
+int bale_hip_set_alc8r(const bale_alc8r_t *a);
 /* The strategy (BALE_STRAT_DIRECT, _LDS or _BUCKET) the last bale_histo,
  * bale_gather or drop-in call on this thread ran; BALE_STRAT_AUTO before any. */
 int bale_hip_last_strategy(void);
