@@ -6,9 +6,8 @@
 // destination's run is written to consecutive output slots.  Per-(chunk,
 // destination) counts from a count pass (rows) and the destinations' first
 // slots (start) place every run, so the output is the input stably
-// partitioned by destination.  Used by the device conveyor (destination =
-// PE number pushed with the item) and by the sparse transpose's radix passes
-// (destination = a digit of the item's first word).
+// partitioned by destination.  Used by the device conveyor and the exstack
+// services (destination = the PE number pushed with the item).
 #pragma once
 #include "bale_internal.h"
 
@@ -79,20 +78,14 @@ inline size_t deliver_lds(int npe) {
 // per PE (LDS), and the waves are prefixed in order, so the ranks follow push
 // order.  rows[c][p]: items of PE p in chunks before c; start[p]: PE p's
 // first slot.
-// KEY 0: destinations from `pes`; KEY 1: destination = (word 0 >> shift) &
-// (npe - 1), npe a power of two (`pes` unused); KEY 2: as 1, and the item is
-// written as the 64-bit value of its word 1 (W = 2: the last radix pass
-// writes the row numbers).
-template <int W, int KEY = 0>
+template <int W>
 __global__ __launch_bounds__(kDlvTile) void k_convey_deliver(const uint32_t* __restrict__ items,
                                                           const int64_t* __restrict__ pes,
                                                           int64_t n, int npe, int nbits,
                                                           int64_t chunk,
                                                           const uint32_t* __restrict__ rows,
                                                           const uint32_t* __restrict__ start,
-                                                          uint32_t* __restrict__ out,
-                                                          int shift = 0) {
-  static_assert(KEY != 2 || W == 2, "KEY 2 writes word 1 of two-word items");
+                                                          uint32_t* __restrict__ out) {
   constexpr int U = deliver_u<W>();
   extern __shared__ __align__(16) uint32_t lds[];
   uint32_t* wcnt = lds;                                  // [kDlvWaves * npe]: count, then offset
@@ -119,7 +112,7 @@ __global__ __launch_bounds__(kDlvTile) void k_convey_deliver(const uint32_t* __r
     pn[u] = -1;
     if (e < hi) {
       item_load<W>(items, (size_t)e, vn[u]);
-      pn[u] = KEY ? (int64_t)((vn[u][0] >> shift) & (uint32_t)(npe - 1)) : pes[e];
+      pn[u] = pes[e];
     }
   }
   for (int64_t base = lo; base < hi; base += (int64_t)U * kDlvTile) {
@@ -135,7 +128,7 @@ __global__ __launch_bounds__(kDlvTile) void k_convey_deliver(const uint32_t* __r
       pn[u] = -1;
       if (e < hi) {
         item_load<W>(items, (size_t)e, vn[u]);
-        pn[u] = KEY ? (int64_t)((vn[u][0] >> shift) & (uint32_t)(npe - 1)) : pes[e];
+        pn[u] = pes[e];
       }
     }
     for (int k = tid; k < kDlvWaves * npe; k += kDlvTile) wcnt[k] = 0;
@@ -198,12 +191,7 @@ __global__ __launch_bounds__(kDlvTile) void k_convey_deliver(const uint32_t* __r
     for (int u = 0; u < U; u++)
       if (d[u] < npe) {
         const uint32_t q = tstart[d[u]] + wc[d[u]] + rank[u];
-        if constexpr (KEY == 2) {
-          const uint32_t r2[2] = {v[u][1], 0u};
-          item_store<W>(stage, q, r2);
-        } else {
-          item_store<W>(stage, q, v[u]);
-        }
+        item_store<W>(stage, q, v[u]);
         sdst[q] = (uint16_t)d[u];
       }
     __syncthreads();

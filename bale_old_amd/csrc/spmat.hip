@@ -193,19 +193,274 @@ __device__ inline int row_of(const int64_t* so, int R, int64_t p) {
   return lo;
 }
 
-// items[j] = (column of nonzero j) | (its row << 32)
-__global__ __launch_bounds__(kThreads) void k_expand(const int64_t* __restrict__ offset,
-                                                     const int64_t* __restrict__ nonzero,
-                                                     int64_t numrows, uint64_t* __restrict__ items) {
-  __shared__ int64_t so[kRowsPerGroup + 1];
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerGroup;
-  const int R = (int)std::min<int64_t>(kRowsPerGroup, numrows - r0);
-  for (int i = threadIdx.x; i <= R; i += kThreads) so[i] = offset[r0 + i];
-  __syncthreads();
-  const int64_t p0 = so[0], p1 = so[R];
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
-    const int r = row_of(so, R, p);
-    items[p] = (uint64_t)(uint32_t)nonzero[p] | ((uint64_t)(r0 + r) << 32);
+// ---- transpose radix passes -----------------------------------------------------------
+// A pass is a stable partition of (column, row) items by one digit of the
+// column.  A workgroup walks its stream in tiles of kRdxTile items; each wave
+// ranks 64 items per round with ballots (one per digit bit) on top of its
+// running count per digit, one block-wide scan over the (digit, wave) counts
+// gives every item its slot in the tile's digit order, the tile is staged in
+// LDS in that order and each digit's run is written to consecutive output
+// slots from the digit's cursor.  8192-item tiles give runs of 32 items
+// (256 bytes) per digit on average.
+//
+// IN 0: items (column | row << 32); IN 1: the matrix's own arrays (column in
+//   the low word of nonzero[]; rows from offset[]: each tile marks the first
+//   position of every row it meets with an LDS max and a max-scan over the
+//   tile's positions gives every position its row).
+// OUT 0: items; OUT 1: the rows, as int64 (the transpose's nonzero array).
+// CUR 0: the stream is chunk blockIdx.x of the input; cursors from a count
+//   pass over sub-chunks of chunk / rstride items (start[p] + rows[c rstride][p]:
+//   the items of digit p before the chunk's first sub-chunk).
+// CUR 1 (any pass after the first): the input is sorted by the low `shift`
+//   bits of the column, so its items with low bits x form group x, in row
+//   order.  Workgroup c takes groups [x0, x1): their items are contiguous, and
+//   its items of digit y are those whose low shift + nbits bits lie in
+//   [y 2^shift + x0, y 2^shift + x1), which the stable pass places in order
+//   from P[y 2^shift + x0], P[v] being the number of items whose low
+//   shift + nbits bits are below v (P indexed up to plen, clamped).  So the
+//   stream and the cursors follow from P with no count pass.  The last pass
+//   uses the transpose's row starts as P; a middle pass a prefix derived from
+//   them (k_low_prefix).
+constexpr int kRdxTile = 8192;  // items per tile
+// Workgroup shapes: 512 threads x 16 items (75 KiB of LDS, two workgroups per
+// CU) when there are enough workgroups to fill the CUs twice, else 1024 x 8.
+template <int NT>
+struct Rdx {
+  static constexpr int U = kRdxTile / NT;
+  static constexpr int NW = NT / 64;
+  // words before the stage: wcnt, tstart, delta, cursor, wsum, srange (16-byte aligned)
+  static constexpr int head = (NW * 256 + 257 + 256 + 256 + NW + 3 + 3) & ~3;
+  static constexpr size_t lds = (size_t)head * 4 + (size_t)kRdxTile * 8;
+};
+constexpr int kRdxPerCu = 2;  // 512-thread delivery workgroups resident per CU
+// count workgroups per delivery chunk
+constexpr int kCountSplit = 2;
+
+__device__ inline int64_t clamp_col(int64_t c, int64_t len) { return c < len ? c : len; }
+
+// P[v], v in [0, M]: nonzeros whose column modulo M is below v, from the row
+// starts of the transpose (M a power of two)
+__global__ __launch_bounds__(256) void k_low_prefix(const int64_t* __restrict__ t_offset, int64_t numcols,
+                                                    int64_t M, int64_t* __restrict__ P) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > M) return;
+  int64_t sum = 0;
+#pragma unroll 8
+  for (int64_t c0 = 0; c0 < numcols; c0 += M)
+    sum += t_offset[clamp_col(c0 + v, numcols)] - t_offset[c0];
+  P[v] = sum;
+}
+
+template <int IN, int OUT, int CUR, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_rdx(const uint64_t* __restrict__ items,
+                                                     const int64_t* __restrict__ offset, int64_t numrows,
+                                                     int64_t n,
+                                                     int npe, int nbits, int shift, int64_t chunk,
+                                                     const uint32_t* __restrict__ rows, int rstride,
+                                                     const uint32_t* __restrict__ start,
+                                                     const int64_t* __restrict__ P, int64_t plen,
+                                                     int64_t ngroups,
+                                                     uint64_t* __restrict__ out,
+                                                     int64_t* __restrict__ out_rows) {
+  constexpr int NW = Rdx<NT>::NW, U = Rdx<NT>::U, T = kRdxTile, K = NW * 256 / NT;
+  extern __shared__ __align__(16) uint32_t lds[];
+  uint32_t* wcnt = lds;                 // [NW * 256] per wave and digit: count, then first staged slot
+  uint32_t* tstart = wcnt + NW * 256;   // [257] first staged slot per digit
+  uint32_t* delta = tstart + 257;       // [256] output slot of staged slot q is delta[p] + q
+  uint32_t* cursor = delta + 256;       // [256] next output slot per digit
+  uint32_t* wsum = cursor + 256;        // [NW] wave totals of the scan
+  uint32_t* srange = wsum + NW;         // [3] stream bounds (CUR 1), first row (IN 1)
+  uint64_t* stage = (uint64_t*)(lds + Rdx<NT>::head);  // [T]
+  const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const uint32_t dmask = (uint32_t)(npe - 1);
+  int64_t lo, hi;
+  if constexpr (CUR == 0) {
+    lo = (int64_t)blockIdx.x * chunk;
+    hi = lo + chunk < n ? lo + chunk : n;
+    for (int p = tid; p < npe; p += NT)
+      cursor[p] = start[p] + rows[(size_t)blockIdx.x * rstride * npe + p];
+  } else {
+    const int64_t x0 = (int64_t)blockIdx.x * ngroups / gridDim.x;
+    const int64_t x1 = ((int64_t)blockIdx.x + 1) * ngroups / gridDim.x;
+    if (tid < 2) srange[tid] = 0;
+    __syncthreads();
+    for (int p = tid; p < npe; p += NT) {
+      const int64_t c0 = (int64_t)p << shift;
+      const int64_t base = P[clamp_col(c0, plen)];
+      const int64_t a = P[clamp_col(c0 + x0, plen)], b = P[clamp_col(c0 + x1, plen)];
+      cursor[p] = (uint32_t)a;
+      atomicAdd(&srange[0], (uint32_t)(a - base));
+      atomicAdd(&srange[1], (uint32_t)(b - base));
+    }
+    __syncthreads();
+    lo = srange[0];
+    hi = srange[1] < n ? (int64_t)srange[1] : n;
+  }
+  const uint64_t below = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+  const int64_t mine = (int64_t)wave * (64 * U) + lane;
+  uint32_t* wc = wcnt + wave * 256;
+  auto load = [&](int64_t e) -> uint64_t {
+    if constexpr (IN == 0) return items[e];
+    else return (uint64_t)(uint32_t)items[e];  // the row is added below
+  };
+  // IN 1: the row holding the stream's first position, by a 64-way search of
+  // offset[] (one wave: each step probes 64 rows and keeps the bracket)
+  uint32_t ra = 0;
+  if constexpr (IN == 1) {
+    if (wave == 0) {
+      int64_t a = 0, b = numrows;  // offset[a] <= lo < offset[b] (b = numrows: offset is nnz > lo)
+      while (b - a > 1) {
+        const int64_t r = a + (b - a) * (lane + 1) / 65;
+        const bool le = r > a && offset[r] <= lo;
+        const uint64_t m = __ballot(le), nm = __ballot(!le && r > a);
+        const int64_t na = m ? __shfl(r, 63 - __clzll(m)) : a;
+        const int64_t nb = nm ? __shfl(r, __builtin_ctzll(nm)) : b;
+        a = na;
+        b = nb;
+      }
+      if (lane == 0) srange[2] = (uint32_t)a;
+    }
+    __syncthreads();
+    ra = srange[2];
+  }
+  for (int64_t base = lo; base < hi; base += T) {
+    // this tile's items; the other workgroup on the CU hides their latency
+    uint64_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t e = base + mine + (int64_t)u * 64;
+      v[u] = e < hi ? load(e) : 0;
+    }
+    if constexpr (IN == 1) {
+      // rows of the tile's positions, in the stage area (free until staging)
+      constexpr int EPT = T / NT;
+      uint32_t* rid = (uint32_t*)stage;
+      for (int k = tid; k < T / 4; k += NT) ((u32x4*)rid)[k] = u32x4{0u, 0u, 0u, 0u};
+      __syncthreads();
+      // every row that starts before the tile's end marks its first position
+      // in the tile (position 0 for the row running into the tile); rows that
+      // share a start (empty ones) leave the largest, the one holding it
+      for (int64_t r = (int64_t)ra + tid; r < numrows; r += NT) {
+        const int64_t o = offset[r];
+        if (o >= base + T) break;
+        atomicMax(&rid[o > base ? (int)(o - base) : 0], (uint32_t)r);
+      }
+      __syncthreads();
+      // inclusive max-scan: thread t owns positions [t EPT, (t + 1) EPT)
+      u32x4* rq = (u32x4*)(rid + tid * EPT);
+      uint32_t m = 0;
+#pragma unroll
+      for (int k = 0; k < EPT / 4; k++) {
+        const u32x4 x = rq[k];
+        m = max(m, max(max(x.x, x.y), max(x.z, x.w)));
+      }
+      uint32_t incl = m;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if (lane >= off) incl = max(incl, o);
+      }
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      uint32_t wpre = lane < wave && lane < NW ? wsum[lane] : 0u;
+#pragma unroll
+      for (int off = 1; off < NW; off <<= 1) wpre = max(wpre, __shfl_xor(wpre, off));
+      wpre = __shfl(wpre, 0);
+      const uint32_t up = __shfl_up(incl, 1);
+      uint32_t run = max(wpre, lane ? up : 0u);
+#pragma unroll
+      for (int k = 0; k < EPT / 4; k++) {
+        u32x4 x = rq[k];
+        x.x = run = max(run, x.x);
+        x.y = run = max(run, x.y);
+        x.z = run = max(run, x.z);
+        x.w = run = max(run, x.w);
+        rq[k] = x;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] |= (uint64_t)rid[mine + u * 64] << 32;
+      ra = rid[T - 1];  // the next tile starts in this row or a later one
+    }
+    for (int k = tid; k < NW * 256; k += NT) wcnt[k] = 0;
+    __syncthreads();
+    // ranks within the wave: lanes with the same digit (one ballot per bit),
+    // after the wave's earlier items of that digit
+    uint32_t rank[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool has = base + mine + (int64_t)u * 64 < hi;
+      const uint32_t du = ((uint32_t)v[u] >> shift) & dmask;
+      uint64_t peers = __ballot(has);
+      for (int bit = 0; bit < nbits; bit++) {
+        const bool one = (du >> bit) & 1;
+        const uint64_t b = __ballot(one);
+        peers &= one ? b : ~b;
+      }
+      const uint32_t r = (uint32_t)__popcll(peers & below);
+      const uint32_t before = has ? wc[du] : 0u;  // every lane reads before the leader adds
+      rank[u] = before + r;
+      if (has && r == 0) wc[du] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // exclusive scan of the counts in (digit, wave) order: thread t holds
+    // entries 4t .. 4t+3 (digit e / NW, wave e % NW); wave scan, then the
+    // wave totals (lanes 0..NW-1 reduce them, lane 0's value is broadcast)
+    {
+      uint32_t cc[K], s = 0;
+      const int e0 = tid * K;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int e = e0 + k, p = e / NW, w = e % NW;
+        cc[k] = p < npe ? wcnt[w * 256 + p] : 0u;
+        s += cc[k];
+      }
+      uint32_t incl = s;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+      }
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      const uint32_t wt = lane < NW ? wsum[lane] : 0u;
+      uint32_t wpre = lane < wave ? wt : 0u, all = wt;
+#pragma unroll
+      for (int off = 1; off < NW; off <<= 1) {
+        wpre += __shfl_xor(wpre, off);
+        all += __shfl_xor(all, off);
+      }
+      wpre = __shfl(wpre, 0);
+      all = __shfl(all, 0);
+      uint32_t run = wpre + incl - s;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int e = e0 + k, p = e / NW, w = e % NW;
+        if (p < npe) {
+          if (w == 0) tstart[p] = run;
+          wcnt[w * 256 + p] = run;
+        }
+        run += cc[k];
+      }
+      if (tid == 0) tstart[npe] = all;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (base + mine + (int64_t)u * 64 < hi) stage[wc[((uint32_t)v[u] >> shift) & dmask] + rank[u]] = v[u];
+    for (int p = tid; p < npe; p += NT) delta[p] = cursor[p] - tstart[p];
+    __syncthreads();
+    // staged order is digit order, input order within a digit
+    const uint32_t ntile = tstart[npe];
+    for (uint32_t q = tid; q < ntile; q += NT) {
+      const uint64_t w = stage[q];
+      const uint32_t g = delta[((uint32_t)w >> shift) & dmask] + q;
+      if (g >= (uint64_t)n) continue;  // only an input the histogram rejected lands here
+      if constexpr (OUT == 0) out[g] = w;
+      else out_rows[g] = (int64_t)(w >> 32);
+    }
+    __syncthreads();
+    for (int p = tid; p < npe; p += NT) cursor[p] = delta[p] + tstart[p + 1];
   }
 }
 
@@ -398,51 +653,88 @@ int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const 
   int bits = 1;
   while (bits < 63 && ((int64_t)1 << bits) < numcols) bits++;
   const int passes = (bits + 7) / 8;
-  // chunks of the item array (one workgroup each, like the conveyor)
   const int64_t n = nnz;
-  // two chunks per CU: two delivery workgroups fit a CU's LDS, and the
-  // second hides the first one's barriers and loads
-  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * 2, ceil_div(std::max<int64_t>(n, 1), kChunkAlign)));
-  const int64_t chunk = ceil_div(ceil_div(std::max<int64_t>(n, 1), C), kChunkAlign) * kChunkAlign;
-  C = ceil_div(std::max<int64_t>(n, 1), chunk);
+  // Chunks of whole tiles: the first pass (rows found in the tile) runs the
+  // 1024-thread shape, one workgroup per CU; later passes with a count, two
+  // 512-thread workgroups per CU.
+  auto chunking = [&](int64_t per_cu, int64_t& C, int64_t& chunk) {
+    C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * per_cu,
+                                               ceil_div(std::max<int64_t>(n, 1), kRdxTile)));
+    chunk = ceil_div(ceil_div(std::max<int64_t>(n, 1), C), kRdxTile) * kRdxTile;
+    C = ceil_div(std::max<int64_t>(n, 1), chunk);
+  };
+  int64_t C1, chunk1, C2, chunk2;
+  chunking(1, C1, chunk1);
+  chunking(kRdxPerCu, C2, chunk2);
   const int Bmax = 256;
-  const size_t head = ((part_words(C, Bmax) * 4) + 255) & ~(size_t)255;
+  const size_t head = ((part_words(std::max(C1, C2) * kCountSplit, Bmax) * 4) + 255) & ~(size_t)255;
   const size_t ib = ((size_t)std::max<int64_t>(n, 1) * 8 + 255) & ~(size_t)255;
   const size_t stb = scan_tmp_bytes(numcols);
-  char* ws = (char*)workspace(d, head + 2 * ib + stb, s);
+  // the middle passes' prefixes: the largest is the second-to-last pass's
+  const int wlast = bits / passes;  // the last (smallest) of the balanced widths
+  const size_t pb = passes >= 3 ? ((((size_t)1 << (bits - wlast)) + 1) * 8 + 255) & ~(size_t)255 : 0;
+  char* ws = (char*)workspace(d, head + 2 * ib + pb + stb, s);
   if (!ws) return (int)hipErrorOutOfMemory;
-  int64_t* stmp = (int64_t*)(ws + head + 2 * ib);
+  int64_t* pbuf = (int64_t*)(ws + head + 2 * ib);
+  int64_t* stmp = (int64_t*)(ws + head + 2 * ib + pb);
   rc = scan_incl(t_offset + 1, numcols, stmp, s);
   if (rc || n == 0) return rc;
   uint64_t* buf[2] = {(uint64_t*)(ws + head), (uint64_t*)(ws + head + ib)};
-  hipLaunchKernelGGL(k_expand, dim3((unsigned)ceil_div(numrows, kRowsPerGroup)), dim3(kThreads), 0, s,
-                     offset, nonzero, numrows, buf[0]);
   int shift = 0;
   for (int k = 0; k < passes; k++) {
     const int w = (bits - shift + (passes - k) - 1) / (passes - k);  // balanced digit widths
     const int B = 1 << w;
-    PartLayout L = part_carve(ws, C, B);
-    const uint32_t* in = (const uint32_t*)buf[k & 1];
-    hipLaunchKernelGGL(k_rx_count, dim3((unsigned)C), dim3(kThreads), 0, s, in, n, shift, B, chunk,
-                       L.chunk);
-    hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk, (int)C, B,
-                       L.total);
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, B, 1u << 30, 0u,
-                       (const uint32_t*)nullptr, L.start, L.segstart, L.misc);
-    const bool last = k == passes - 1;
-    void* out = last ? (void*)t_nonzero : (void*)buf[(k + 1) & 1];
-    const size_t lds = deliver_lds<2>(B);
-    if (last) {
-      allow_lds(k_convey_deliver<2, 2>, lds);
-      hipLaunchKernelGGL((k_convey_deliver<2, 2>), dim3((unsigned)C), dim3(kDlvTile), lds, s, in,
-                         (const int64_t*)nullptr, n, B, w, chunk, (const uint32_t*)L.chunk,
-                         (const uint32_t*)L.start, (uint32_t*)out, shift);
-    } else {
-      allow_lds(k_convey_deliver<2, 1>, lds);
-      hipLaunchKernelGGL((k_convey_deliver<2, 1>), dim3((unsigned)C), dim3(kDlvTile), lds, s, in,
-                         (const int64_t*)nullptr, n, B, w, chunk, (const uint32_t*)L.chunk,
-                         (const uint32_t*)L.start, (uint32_t*)out, shift);
+    const bool first = k == 0, last = k == passes - 1;
+    // the first pass reads the matrix itself; the last one places the rows
+    // by column offsets when there are enough groups to spread
+    const uint64_t* in = first ? (const uint64_t*)nonzero : buf[k & 1];
+    uint64_t* out = buf[(k + 1) & 1];
+    // after the first pass, a pass with at least a group per CU needs no count
+    const int64_t ngroups = (int64_t)1 << shift;
+    const bool by_groups = !first && ngroups >= (int64_t)d->cus;
+    const int64_t* P = t_offset;
+    int64_t plen = numcols;
+    if (by_groups && !last) {
+      const int64_t M = (int64_t)1 << (shift + w);
+      hipLaunchKernelGGL(k_low_prefix, dim3((unsigned)ceil_div(M + 1, 256)), dim3(256), 0, s, t_offset,
+                         numcols, M, pbuf);
+      P = pbuf;
+      plen = M;
     }
+    const int64_t C = first ? C1 : C2, chunk = first ? chunk1 : chunk2;
+    PartLayout L = part_carve(ws, C * kCountSplit, B);
+    if (!by_groups) {
+      hipLaunchKernelGGL(k_rx_count, dim3((unsigned)(C * kCountSplit)), dim3(kThreads), 0, s,
+                         (const uint32_t*)in, n, shift, B, chunk / kCountSplit, L.chunk);
+      hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)B), dim3(256), 0, s, L.chunk,
+                         (int)(C * kCountSplit), B, L.total);
+      hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, L.total, B, 1u << 30, 0u,
+                         (const uint32_t*)nullptr, L.start, L.segstart, L.misc);
+    }
+    const uint32_t* cr = L.chunk;
+    const uint32_t* cs = L.start;
+#define BALE_RDX(IN, OUT, CUR, G)                                                                       \
+  do {                                                                                                  \
+    if ((G) >= (int64_t)d->cus * kRdxPerCu) {                                                           \
+      allow_lds(k_rdx<IN, OUT, CUR, 512>, Rdx<512>::lds);                                               \
+      hipLaunchKernelGGL((k_rdx<IN, OUT, CUR, 512>), dim3((unsigned)(G)), dim3(512), Rdx<512>::lds, s,  \
+                         in, offset, numrows, n, B, w, shift, chunk, cr, kCountSplit, cs, P, plen, ngroups, out,  \
+                         t_nonzero);                                                                    \
+    } else {                                                                                            \
+      allow_lds(k_rdx<IN, OUT, CUR, 1024>, Rdx<1024>::lds);                                             \
+      hipLaunchKernelGGL((k_rdx<IN, OUT, CUR, 1024>), dim3((unsigned)(G)), dim3(1024), Rdx<1024>::lds, s, \
+                         in, offset, numrows, n, B, w, shift, chunk, cr, kCountSplit, cs, P, plen, ngroups, out,  \
+                         t_nonzero);                                                                    \
+    }                                                                                                   \
+  } while (0)
+    const int64_t G = std::min<int64_t>(ngroups, (int64_t)d->cus * kRdxPerCu);
+    if (first && last) BALE_RDX(1, 1, 0, C);
+    else if (first) BALE_RDX(1, 0, 0, C);
+    else if (by_groups && last) BALE_RDX(0, 1, 1, G);
+    else if (by_groups) BALE_RDX(0, 0, 1, G);
+    else if (last) BALE_RDX(0, 1, 0, C);
+    else BALE_RDX(0, 0, 0, C);
+#undef BALE_RDX
     rc = (int)hipGetLastError();
     if (rc) return rc;
     shift += w;

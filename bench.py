@@ -525,6 +525,7 @@ WORKLOADS = {
     "zipf": "the same plan on 2^28 Zipf(1.1) updates (its sample switches the hot table on)",
     "small": "the planned histogram of 2^24 uniform updates into 2^20 int64 bins",
     "batch": "the held-out mix of 4096 instances (seed 5), learned choices, launched three times",
+    "transpose": "three transposes of the Erdos-Renyi matrix of 2^24 rows (p = 18 / n, both triangles)",
 }
 
 
@@ -563,6 +564,13 @@ def run_workloads(which) -> int:
         B.gen_index(idx[:m], nb, seed=1)
         B.histo(idx[:m], counts, cfg=B.histo_plan(m, nb))
         del counts
+    if "transpose" in which:
+        nr = 1 << 24
+        off, nz = B.spmat_gen_er(nr, 18 / nr, 0, 3, seed=101894)
+        for _ in range(3):
+            toff, tnz = B.spmat_transpose(nr, off, nz)
+        torch.cuda.synchronize()
+        del off, nz, toff, tnz
     if "batch" in which:
         del idx
         from bale_old_amd import mix as MX, selector as S

@@ -328,6 +328,30 @@ def test_transpose_and_permute_random_shapes(dev, orc, seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("numcols", [3, 1 << 16, 1 << 20, 1 << 24])
+def test_transpose_rows_across_tiles(dev, orc, numcols):
+    """Enough nonzeros for many 8192-item tiles per chunk, with rows that run
+    across tile and chunk boundaries: a stretch of 100000 empty rows, one row
+    of 300000 nonzeros, single-nonzero rows, repeated columns.  numcols picks
+    one pass (3), two (2^16: the last pass by column groups), three with a
+    counted middle pass (2^20) and three with both later passes by groups
+    (2^24)."""
+    rng = np.random.default_rng(77 + numcols)
+    numrows = 4_000_000
+    lens = rng.poisson(5.0, numrows)
+    lens[1_000_000:1_100_000] = 0
+    lens[2_500_000] = 300_000
+    lens[3_000_000:3_200_000] = 1
+    off = np.zeros(numrows + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    nz = rng.integers(0, numcols, off[-1], dtype=np.int64)
+    toff, tnz = B.spmat_transpose(numcols, _dev(off), _dev(nz))
+    woff, wnz = orc.transpose_agi(numrows, numcols, off, nz)
+    assert np.array_equal(toff.cpu().numpy(), woff)
+    assert np.array_equal(tnz.cpu().numpy(), wnz)
+
+
+@pytest.mark.gpu
 def test_transpose_er_and_back(dev, orc):
     n = 1 << 21
     off, nz = B.spmat_gen_er(n, 18 / n, 0, 3, seed=101894)
