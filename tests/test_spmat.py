@@ -373,7 +373,9 @@ def test_transpose_rejects_bad_columns(dev):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,mean", [(1, 1), (777, 5), (100000, 9)])
+# the last two take the column map through the library's gather (at least
+# 2^24 nonzeros): by table region at 2^22 columns, one load per request at 2^21
+@pytest.mark.parametrize("n,mean", [(1, 1), (777, 5), (100000, 9), (1 << 22, 5), (1 << 21, 12)])
 def test_permute_matches_oracle(dev, orc, n, mean):
     rng = np.random.default_rng(n)
     lens = rng.poisson(mean, n)
@@ -385,6 +387,23 @@ def test_permute_matches_oracle(dev, orc, n, mean):
     woff, wnz = orc.permute_agi(off, nz, rpi, cpi)
     assert np.array_equal(poff.cpu().numpy(), woff)
     assert np.array_equal(pnz.cpu().numpy(), wnz)
+
+
+@pytest.mark.gpu
+def test_permute_reports_a_bad_column_in_a_large_matrix(dev):
+    """The gather that maps the columns of a large matrix counts a column out
+    of range like the small path does."""
+    n = 1 << 22
+    rng = np.random.default_rng(5)
+    lens = rng.poisson(5, n)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    nz = rng.integers(0, n, off[-1], dtype=np.int64)
+    nz[12345] = n + 7
+    rpi, cpi = rng.permutation(n).astype(np.int64), rng.permutation(n).astype(np.int64)
+    with pytest.raises(B.BaleError) as e:
+        B.spmat_permute(n, _dev(off), _dev(nz), _dev(rpi), _dev(cpi))
+    assert e.value.code == -2
 
 
 @pytest.mark.gpu
