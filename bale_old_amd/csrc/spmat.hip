@@ -56,7 +56,8 @@ __device__ inline int64_t block_incl_scan(int64_t v, int64_t* sh, int64_t* total
 }
 
 // sums[t] = sum of a[t * kScanTile ..) (thread tid owns kPer consecutive values)
-__global__ __launch_bounds__(kThreads) void k_scan_sums(const int64_t* __restrict__ a, int64_t n,
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_scan_sums(const T* __restrict__ a, int64_t n,
                                                         int64_t* __restrict__ sums) {
   __shared__ int64_t sh[kThreads / 64];
   const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kPer;
@@ -101,6 +102,33 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(int64_t* __restrict__ a
     run += x[k];
     if (b + k < n) a[b + k] = run;
   }
+}
+
+// out[i] = sum of in[0..i] (inclusive) for 32-bit counts, into int64
+__global__ __launch_bounds__(kThreads) void k_scan_apply32(const int32_t* __restrict__ in, int64_t n,
+                                                           const int64_t* __restrict__ sums,
+                                                           int64_t* __restrict__ out) {
+  __shared__ int64_t sh[kThreads / 64];
+  const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kPer;
+  int64_t x[kPer], v = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    x[k] = b + k < n ? in[b + k] : 0;
+    v += x[k];
+  }
+  int64_t total;
+  int64_t run = block_incl_scan(v, sh, &total) - v + sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    run += x[k];
+    if (b + k < n) out[b + k] = run;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_fill_i64(int64_t* __restrict__ p, int64_t n, int64_t v) {
@@ -223,7 +251,9 @@ __device__ inline int row_of(const int64_t* so, int R, int64_t p) {
 //   them (k_low_prefix).
 constexpr int kRdxTile = 8192;  // items per tile
 // Workgroup shapes: 512 threads x 16 items (75 KiB of LDS, two workgroups per
-// CU) when there are enough workgroups to fill the CUs twice, else 1024 x 8.
+// CU) when there are enough workgroups to fill the CUs twice, else 1024 x 8
+// (a pass by column groups with fewer groups than that: at 2^24 columns the
+// middle pass has 256).
 template <int NT>
 struct Rdx {
   static constexpr int U = kRdxTile / NT;
@@ -325,11 +355,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   }
   for (int64_t base = lo; base < hi; base += T) {
     // this tile's items; the other workgroup on the CU hides their latency
+    // (IN 1 loads them once the rows are known, so the row pass runs with
+    // the registers free)
     uint64_t v[U];
+    if constexpr (IN == 0) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int64_t e = base + mine + (int64_t)u * 64;
-      v[u] = e < hi ? load(e) : 0;
+      for (int u = 0; u < U; u++) {
+        const int64_t e = base + mine + (int64_t)u * 64;
+        v[u] = e < hi ? load(e) : 0;
+      }
     }
     if constexpr (IN == 1) {
       // rows of the tile's positions, in the stage area (free until staging)
@@ -379,7 +413,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       }
       __syncthreads();
 #pragma unroll
-      for (int u = 0; u < U; u++) v[u] |= (uint64_t)rid[mine + u * 64] << 32;
+      for (int u = 0; u < U; u++) {
+        const int64_t e = base + mine + (int64_t)u * 64;
+        v[u] = e < hi ? load(e) | (uint64_t)rid[mine + u * 64] << 32 : 0;
+      }
       ra = rid[T - 1];  // the next tile starts in this row or a later one
     }
     for (int k = tid; k < NW * 256; k += NT) wcnt[k] = 0;
@@ -610,9 +647,19 @@ static int scan_incl(int64_t* a, int64_t n, int64_t* tmp, hipStream_t s) {
   using namespace sm;
   if (n <= 0) return 0;
   const int64_t nt = ceil_div(n, kScanTile);
-  hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nt), dim3(kThreads), 0, s, (const int64_t*)a, n, tmp);
+  hipLaunchKernelGGL(k_scan_sums<int64_t>, dim3((unsigned)nt), dim3(kThreads), 0, s, (const int64_t*)a, n, tmp);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, tmp, nt);
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kThreads), 0, s, a, n, (const int64_t*)tmp);
+  return (int)hipGetLastError();
+}
+// out[i] = sum of in[0..i] for int32 counts; tmp as for scan_incl.
+static int scan_incl32(const int32_t* in, int64_t n, int64_t* out, int64_t* tmp, hipStream_t s) {
+  using namespace sm;
+  if (n <= 0) return 0;
+  const int64_t nt = ceil_div(n, kScanTile);
+  hipLaunchKernelGGL(k_scan_sums<int32_t>, dim3((unsigned)nt), dim3(kThreads), 0, s, in, n, tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, tmp, nt);
+  hipLaunchKernelGGL(k_scan_apply32, dim3((unsigned)nt), dim3(kThreads), 0, s, in, n, (const int64_t*)tmp, out);
   return (int)hipGetLastError();
 }
 static size_t scan_tmp_bytes(int64_t n) {
@@ -644,13 +691,24 @@ int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const 
                            const int64_t* nonzero, int64_t* t_offset, int64_t* t_nonzero,
                            Device* d, hipStream_t s) {
   using namespace sm;
-  // row starts of the transpose: column counts, then their prefix
-  hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(numcols + 1, 256, d)), dim3(256), 0, s, t_offset,
-                     numcols + 1, (int64_t)0);
+  // row starts of the transpose: column counts, then their prefix.  The
+  // counts fit 32 bits (nnz < 2^31); they go to the front of t_nonzero, which
+  // only the last radix pass writes, when they fit there (numcols <= 2 nnz),
+  // else straight into t_offset as int64.
+  const bool c32 = nnz > 0 && numcols <= 2 * nnz;
+  int32_t* cnt32 = (int32_t*)t_nonzero;
+  if (c32) {
+    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, t_offset, (int64_t)1, (int64_t)0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(numcols, 256, d)), dim3(256), 0, s, cnt32, numcols);
+  } else {
+    hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(numcols + 1, 256, d)), dim3(256), 0, s, t_offset,
+                       numcols + 1, (int64_t)0);
+  }
   int rc = (int)hipGetLastError();
   if (rc) return rc;
   if (nnz > 0) {
-    rc = bale_histo(nonzero, nnz, t_offset + 1, numcols, BALE_I64, BALE_INDEX_PLAIN, nullptr, s);
+    rc = c32 ? bale_histo(nonzero, nnz, cnt32, numcols, BALE_I32, BALE_INDEX_PLAIN, nullptr, s)
+             : bale_histo(nonzero, nnz, t_offset + 1, numcols, BALE_I64, BALE_INDEX_PLAIN, nullptr, s);
     if (rc) return rc;  // a BALE_ERR_* code with the error text set
   }
   // digits: LSD passes of at most 8 bits over the column
@@ -658,20 +716,15 @@ int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const 
   while (bits < 63 && ((int64_t)1 << bits) < numcols) bits++;
   const int passes = (bits + 7) / 8;
   const int64_t n = nnz;
-  // Chunks of whole tiles: the first pass (rows found in the tile) runs the
-  // 1024-thread shape, one workgroup per CU; later passes with a count, two
-  // 512-thread workgroups per CU.
-  auto chunking = [&](int64_t per_cu, int64_t& C, int64_t& chunk) {
-    C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * per_cu,
-                                               ceil_div(std::max<int64_t>(n, 1), kRdxTile)));
-    chunk = ceil_div(ceil_div(std::max<int64_t>(n, 1), C), kRdxTile) * kRdxTile;
-    C = ceil_div(std::max<int64_t>(n, 1), chunk);
-  };
-  int64_t C1, chunk1, C2, chunk2;
-  chunking(1, C1, chunk1);
-  chunking(kRdxPerCu, C2, chunk2);
+  // Passes with a count: chunks of whole tiles, two delivery workgroups per CU
+  // (the 512-thread shape, also for the first pass: it measured 6.5 against
+  // 6.85 ms for the call with the first pass on one 1024-thread workgroup per CU).
+  int64_t C = std::max<int64_t>(1, std::min<int64_t>((int64_t)d->cus * kRdxPerCu,
+                                                     ceil_div(std::max<int64_t>(n, 1), kRdxTile)));
+  const int64_t chunk = ceil_div(ceil_div(std::max<int64_t>(n, 1), C), kRdxTile) * kRdxTile;
+  C = ceil_div(std::max<int64_t>(n, 1), chunk);
   const int Bmax = 256;
-  const size_t head = ((part_words(std::max(C1, C2) * kCountSplit, Bmax) * 4) + 255) & ~(size_t)255;
+  const size_t head = ((part_words(C * kCountSplit, Bmax) * 4) + 255) & ~(size_t)255;
   const size_t ib = ((size_t)std::max<int64_t>(n, 1) * 8 + 255) & ~(size_t)255;
   const size_t stb = scan_tmp_bytes(numcols);
   // the middle passes' prefixes: the largest is the second-to-last pass's
@@ -681,7 +734,7 @@ int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const 
   if (!ws) return (int)hipErrorOutOfMemory;
   int64_t* pbuf = (int64_t*)(ws + head + 2 * ib);
   int64_t* stmp = (int64_t*)(ws + head + 2 * ib + pb);
-  rc = scan_incl(t_offset + 1, numcols, stmp, s);
+  rc = c32 ? scan_incl32(cnt32, numcols, t_offset + 1, stmp, s) : scan_incl(t_offset + 1, numcols, stmp, s);
   if (rc || n == 0) return rc;
   uint64_t* buf[2] = {(uint64_t*)(ws + head), (uint64_t*)(ws + head + ib)};
   int shift = 0;
@@ -705,7 +758,6 @@ int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const 
       P = pbuf;
       plen = M;
     }
-    const int64_t C = first ? C1 : C2, chunk = first ? chunk1 : chunk2;
     PartLayout L = part_carve(ws, C * kCountSplit, B);
     if (!by_groups) {
       hipLaunchKernelGGL(k_rx_count, dim3((unsigned)(C * kCountSplit)), dim3(kThreads), 0, s,
