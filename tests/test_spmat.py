@@ -352,6 +352,26 @@ def test_transpose_rows_across_tiles(dev, orc, numcols):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("numrows,mean", [(3_000_000, 5.0), (400_000, 5.0)])
+def test_transpose_four_passes(dev, orc, numrows, mean):
+    """2^25 + 3 columns: four radix passes (7, 7, 6, 6 bits) -- a counted
+    middle pass (128 groups), a middle pass by groups (16384 groups, its
+    prefix over columns modulo 2^20) and the last by groups.  The first case
+    counts the columns in int32 inside the output array; the second has more
+    than twice as many columns as nonzeros and counts them as int64."""
+    numcols = (1 << 25) + 3
+    rng = np.random.default_rng(numrows)
+    lens = rng.poisson(mean, numrows)
+    off = np.zeros(numrows + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    nz = rng.integers(0, numcols, off[-1], dtype=np.int64)
+    toff, tnz = B.spmat_transpose(numcols, _dev(off), _dev(nz))
+    woff, wnz = orc.transpose_agi(numrows, numcols, off, nz)
+    assert np.array_equal(toff.cpu().numpy(), woff)
+    assert np.array_equal(tnz.cpu().numpy(), wnz)
+
+
+@pytest.mark.gpu
 def test_transpose_er_and_back(dev, orc):
     n = 1 << 21
     off, nz = B.spmat_gen_er(n, 18 / n, 0, 3, seed=101894)
