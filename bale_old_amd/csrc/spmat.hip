@@ -693,9 +693,12 @@ int launch_spmat_transpose(int64_t numrows, int64_t numcols, int64_t nnz, const 
   using namespace sm;
   // row starts of the transpose: column counts, then their prefix.  The
   // counts fit 32 bits (nnz < 2^31); they go to the front of t_nonzero, which
-  // only the last radix pass writes, when they fit there (numcols <= 2 nnz),
-  // else straight into t_offset as int64.
-  const bool c32 = nnz > 0 && numcols <= 2 * nnz;
+  // only the last radix pass writes, when they fit there (numcols <= 2 nnz)
+  // and t_nonzero is not the matrix's own array (a caller may transpose a
+  // square matrix's nonzeros in place), else straight into t_offset as int64.
+  const bool overlap = (const void*)t_nonzero < (const void*)(nonzero + nnz) &&
+                       (const void*)nonzero < (const void*)(t_nonzero + nnz);
+  const bool c32 = nnz > 0 && numcols <= 2 * nnz && !overlap;
   int32_t* cnt32 = (int32_t*)t_nonzero;
   if (c32) {
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, t_offset, (int64_t)1, (int64_t)0);

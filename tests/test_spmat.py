@@ -372,6 +372,20 @@ def test_transpose_four_passes(dev, orc, numrows, mean):
 
 
 @pytest.mark.gpu
+def test_transpose_in_place(dev, orc):
+    """t_nonzero may be the matrix's own nonzero array (square matrices): the
+    result equals the transpose into fresh arrays."""
+    import torch
+    n = 1 << 20
+    off, nz = B.spmat_gen_er(n, 18 / n, 0, 3, seed=7)
+    want_off, want_nz = B.spmat_transpose(n, off, nz)
+    toff = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    B.check(B.lib().bale_spmat_transpose(n, n, nz.numel(), off.data_ptr(), nz.data_ptr(),
+                                         toff.data_ptr(), nz.data_ptr(), None))
+    assert torch.equal(toff, want_off) and torch.equal(nz, want_nz)
+
+
+@pytest.mark.gpu
 def test_transpose_er_and_back(dev, orc):
     n = 1 << 21
     off, nz = B.spmat_gen_er(n, 18 / n, 0, 3, seed=101894)
