@@ -93,6 +93,33 @@ def test_plain_gather_full(dev, orc):
     torch.cuda.empty_cache()
 
 
+# ---- beyond 2^30: the bucketed passes run in slices ---------------------------
+@pytest.mark.gpu
+def test_calls_past_2p30_run_in_slices(dev):
+    """A gather and a histogram of 2^30 + 12345 requests / updates: the
+    bucketed passes keep 32-bit positions and run in slices of 2^30, so the
+    boundary falls inside both calls.  Checked element for element (splitmix64
+    in torch; torch.bincount)."""
+    n, tlen = (1 << 30) + 12345, 1 << 26
+    idx = torch.empty(n, dtype=torch.int64, device=dev)
+    B.gen_index(idx, tlen, seed=11)
+    table = torch.empty(tlen, dtype=torch.int64, device=dev)
+    B.fill_splitmix(table)
+    tgt = torch.empty(n, dtype=torch.int64, device=dev)
+    tgt.fill_(-7)
+    B.gather(tgt, idx, table, cfg=B.config(B.BUCKET))
+    assert B.lib().bale_hip_last_strategy() == B.BUCKET
+    assert TC.gather_matches_splitmix(tgt, idx)
+    del tgt, table
+    torch.cuda.empty_cache()
+    counts = torch.zeros(tlen, dtype=torch.int64, device=dev)
+    B.histo(idx, counts, cfg=B.config(B.BUCKET, block=1024, hot=2))
+    assert B.lib().bale_hip_last_strategy() == B.BUCKET
+    assert torch.equal(counts, torch.bincount(idx, minlength=tlen))
+    del idx, counts
+    torch.cuda.empty_cache()
+
+
 # ---- config: skewed histogram, 2^28 Zipf(1.1) updates into 2^26 bins -----------
 @pytest.mark.gpu
 @pytest.mark.parametrize("run_mean", [0, 64], ids=["zipf1.1", "zipf1.1_runs64"])
