@@ -157,6 +157,9 @@ int launch_histo_lds(const int64_t* idx, int64_t n, void* counts, int64_t nbins,
 int launch_histo_bucket(const int64_t* idx, int64_t n, void* counts, int64_t nbins, int dtype,
                         bool packed, const bale_config& c, Device* d, hipStream_t s);
 
+// offT[r][c] = starts[c][r] for u16 rows of `rows` entries, C rows (gather.hip)
+int launch_starts_transpose(const uint16_t* starts, uint16_t* offT, int rows, int64_t C, hipStream_t s);
+
 // random permutation by dart throwing (randperm.hip); darts_used (optional)
 // receives the darts generated
 int launch_randperm(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream_t s,

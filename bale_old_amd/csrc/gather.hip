@@ -364,6 +364,14 @@ __global__ __launch_bounds__(1024) void k_gunpermute(const uint32_t* __restrict_
   }
 }
 
+int launch_starts_transpose(const uint16_t* starts, uint16_t* offT, int rows, int64_t C, hipStream_t s) {
+  if (C <= 0 || rows <= 0) return 0;
+  if (ceil_div(C, 64) > 65535) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gtranspose, dim3((unsigned)ceil_div(rows, 64), (unsigned)ceil_div(C, 64)), dim3(256), 0, s,
+                     starts, offT, rows, C, (const uint32_t*)nullptr);
+  return (int)hipGetLastError();
+}
+
 template <bool PACKED, bool VEC>
 static void go_direct(int U, dim3 g, dim3 b, hipStream_t s, int64_t* tgt, const int64_t* idx,
                       int64_t n, const int64_t* table, uint64_t tlen, ErrWords* err,

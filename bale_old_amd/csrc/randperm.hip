@@ -213,6 +213,327 @@ __global__ __launch_bounds__(kThreads) void k_compact(const uint32_t* __restrict
   }
 }
 
+// ---- the same darts resolved by slot region, in LDS -----------------------
+// For up to kRsMaxRegions regions of 2^15 slots (M <= 2^27) no dart meets a
+// device atomic or a random global access:
+//   k_rs_sort   one workgroup per chunk of 2^14 darts generates them and writes
+//               them grouped by region as u32 entries (slot in region << 14 |
+//               dart in chunk), with the chunk's run starts (one u16 row)
+//   (the run starts transposed to one row per region)
+//   k_rs_first  one workgroup per region walks its runs chunk by chunk: an LDS
+//               atomic min per slot gives each slot's first dart, a second walk
+//               flags the entries whose dart is that first one (a byte each)
+//   k_rs_rank   one workgroup per chunk: the flagged darts' ranks in dart order
+//               within the chunk (u16 per entry) and the chunk's count
+//   k_rs_place  one workgroup per region: each slot's item (the chunk's offset
+//               plus the rank, empty from item N on) in slot order, with the
+//               filled slots counted per tile
+// and k_scan_tiles + k_compact list the filled slots as before.
+constexpr int kRsChunkBits = 14;
+constexpr uint32_t kRsChunk = 1u << kRsChunkBits;  // darts per chunk (= kTile)
+constexpr int kRsRegionBits = 15;
+constexpr uint32_t kRsRegion = 1u << kRsRegionBits;  // slots per region (2 tiles)
+constexpr int kRsMaxRegions = 4096;
+constexpr uint16_t kNoRank = 0xFFFF;
+static_assert(kRsChunk == (uint32_t)kTile && kRsRegion == 2 * (uint32_t)kTile, "tiles and regions");
+
+// Region served by workgroup b: consecutive regions on one XCD (workgroup b
+// runs on XCD b % 8), so the entry lines shared by neighbouring regions' runs
+// are read into one L2.
+__device__ inline int rs_region(unsigned b, int B) {
+  const int per = (B + 7) / 8;
+  return (int)(b % 8) * per + (int)(b / 8);
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_sort(uint64_t x0, uint64_t D, uint32_t M, int B,
+                                                      uint32_t* __restrict__ ent,
+                                                      uint16_t* __restrict__ starts) {
+  extern __shared__ uint32_t sh[];
+  uint32_t* cnt = sh;                     // [B] counts, then run starts
+  uint32_t* stage = sh + ((B + 3) & ~3);  // [kRsChunk] entries grouped by region
+  __shared__ uint32_t s_wave[kThreads / 64];
+  __shared__ uint32_t s_valid;
+  const int tid = threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * kRsChunk;
+  for (int r = tid; r < B; r += kThreads) cnt[r] = 0;
+  // darts base + 16 tid + i, as k_darts makes them
+  const uint64_t k0 = base + (uint64_t)tid * kDartsPerThread;
+  uint32_t e[kDartsPerThread], reg[kDartsPerThread], rank[kDartsPerThread];
+  const Lcg j = lcg_steps(k0);
+  uint64_t x = (j.a * x0 + j.c) & kMask48;
+#pragma unroll
+  for (int i = 0; i < kDartsPerThread; i++) {
+    x = (kA * x + kC) & kMask48;
+    const uint32_t s = (uint32_t)(x >> 17) % M;
+    reg[i] = k0 + i < D ? s >> kRsRegionBits : ~0u;
+    e[i] = ((s & (kRsRegion - 1)) << kRsChunkBits) | (uint32_t)(tid * kDartsPerThread + i);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kDartsPerThread; i++) rank[i] = reg[i] != ~0u ? atomicAdd(&cnt[reg[i]], 1u) : 0u;
+  __syncthreads();
+  // exclusive scan of cnt[0..B): each thread sums a run of up to 4 counters
+  const int per = (B + kThreads - 1) / kThreads;
+  const int r0 = tid * per;
+  uint32_t mine = 0;
+  for (int q = 0; q < per; q++)
+    if (r0 + q < B) mine += cnt[r0 + q];
+  uint32_t incl = mine;
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) s_wave[wv] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int q = 0; q < kThreads / 64; q++) {
+      const uint32_t t = s_wave[q];
+      s_wave[q] = run;
+      run += t;
+    }
+    s_valid = run;
+  }
+  __syncthreads();
+  uint32_t run = s_wave[wv] + incl - mine;
+  for (int q = 0; q < per; q++)
+    if (r0 + q < B) {
+      const uint32_t c = cnt[r0 + q];
+      cnt[r0 + q] = run;
+      run += c;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kDartsPerThread; i++)
+    if (reg[i] != ~0u) stage[cnt[reg[i]] + rank[i]] = e[i];
+  const uint32_t valid = s_valid;
+  uint16_t* row = starts + (size_t)blockIdx.x * (B + 1);
+  for (int r = tid; r < B; r += kThreads) row[r] = (uint16_t)cnt[r];
+  if (tid == 0) row[B] = (uint16_t)valid;
+  __syncthreads();
+  uint32_t* out = ent + base;
+  for (uint32_t q = tid; q < valid / 4; q += kThreads) ((u32x4*)out)[q] = ((const u32x4*)stage)[q];
+  for (uint32_t q = (valid & ~3u) + tid; q < valid; q += kThreads) out[q] = stage[q];
+}
+
+// Walk of region r's entries by the workgroup's waves, each on its own: wave w
+// takes the groups of 64 chunks g = w, w + 16, ...; lane l holds chunk
+// 64 g + l's run (its start and the entries of the group's earlier runs), and
+// the group's entries f = 0, 1, ... go to lanes in turn, Q per lane at a time
+// (consecutive entries on consecutive lanes, so a load instruction touches
+// about one line per run).  An entry's run is the last lane whose prefix is
+// <= f: its block of 8 lanes from the eight block prefixes held in scalar
+// registers, then three cross-lane steps inside the block.  The next group's
+// run starts are loaded while the current one is served.  No barrier: the
+// waves meet only where the caller syncs.  setup(chunk) runs on each lane for
+// its chunk (chunk >= C past the end); fn(pos[Q], val[Q]) gets Q entry
+// positions (-1: none) and setup's value for each one's chunk.
+template <int Q, typename Setup, typename Fn>
+__device__ inline void rs_walk(const uint16_t* __restrict__ offT, int64_t C, int r, Setup setup, Fn fn) {
+  constexpr int kWaves = kThreads / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t ng = (C + 63) / 64;
+  const uint16_t* rowa = offT + (int64_t)r * C;
+  const uint16_t* rowb = offT + (int64_t)(r + 1) * C;
+  uint32_t na = 0, nb = 0;
+  {
+    const int64_t c = (int64_t)wv * 64 + lane;
+    if (c < C) {
+      na = rowa[c];
+      nb = rowb[c];
+    }
+  }
+  for (int64_t g = wv; g < ng; g += kWaves) {
+    const int64_t c = g * 64 + lane;
+    const uint32_t a = na, len = nb - na;
+    const int64_t cn = c + (int64_t)kWaves * 64;
+    na = nb = 0;
+    if (cn < C) {
+      na = rowa[cn];
+      nb = rowb[cn];
+    }
+    const uint32_t val = setup(c);
+    uint32_t incl = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    const uint32_t pre = incl - len;  // lanes past the group's end: pre = total
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    uint32_t bp[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) bp[k] = (uint32_t)__builtin_amdgcn_readlane((int)pre, 8 * k);
+    // entry f of run l sits at chunk l's first entry + a_l + (f - pre_l)
+    const int32_t delta = (int32_t)(a - pre);
+    const int64_t gbase = g * 64 * (int64_t)kRsChunk;
+    for (uint32_t f0 = 0; f0 < total; f0 += Q * 64) {
+      int64_t pos[Q];
+      uint32_t v[Q];
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const uint32_t f = f0 + (uint32_t)(q * 64 + lane);
+        int lo = 0;
+#pragma unroll
+        for (int k = 1; k < 8; k++) lo += bp[k] <= f ? 8 : 0;
+#pragma unroll
+        for (int step = 4; step >= 1; step >>= 1)
+          if ((uint32_t)__shfl((int)pre, lo + step) <= f) lo += step;
+        const int32_t dl = __shfl(delta, lo);
+        v[q] = (uint32_t)__shfl((int)val, lo);
+        pos[q] = f < total ? gbase + (int64_t)lo * kRsChunk + (int64_t)f + dl : -1;
+      }
+      fn(pos, v);
+    }
+  }
+}
+
+// first dart of each slot of the region, then a flag per entry: its dart is first
+__global__ __launch_bounds__(kThreads) void k_rs_first(const uint32_t* __restrict__ ent,
+                                                       const uint16_t* __restrict__ offT, int64_t C, int B,
+                                                       uint8_t* __restrict__ hit) {
+  extern __shared__ uint32_t tbl[];  // [kRsRegion] smallest dart per slot
+  const int r = rs_region(blockIdx.x, B);
+  if (r >= B) return;
+  for (uint32_t i = threadIdx.x; i < kRsRegion; i += kThreads) tbl[i] = kEmpty;
+  __syncthreads();
+  constexpr int U = 4;
+  auto none = [](int64_t) { return 0u; };
+  rs_walk<U>(offT, C, r, none, [&](const int64_t* pos, const uint32_t*) {
+    uint32_t v[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) v[q] = pos[q] >= 0 ? ent[pos[q]] : 0u;
+#pragma unroll
+    for (int q = 0; q < U; q++)
+      if (pos[q] >= 0) {
+        const uint32_t k = (uint32_t)(pos[q] & ~(int64_t)(kRsChunk - 1)) | (v[q] & (kRsChunk - 1));
+        atomicMin(&tbl[v[q] >> kRsChunkBits], k);
+      }
+  });
+  __syncthreads();  // every minimum is in place
+  rs_walk<U>(offT, C, r, none, [&](const int64_t* pos, const uint32_t*) {
+    uint32_t v[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) v[q] = pos[q] >= 0 ? ent[pos[q]] : 0u;
+#pragma unroll
+    for (int q = 0; q < U; q++)
+      if (pos[q] >= 0) {
+        const uint32_t k = (uint32_t)(pos[q] & ~(int64_t)(kRsChunk - 1)) | (v[q] & (kRsChunk - 1));
+        hit[pos[q]] = tbl[v[q] >> kRsChunkBits] == k ? 1 : 0;
+      }
+  });
+}
+
+// Ranks of the chunk's first darts in dart order: per entry (kNoRank for the
+// others) and the chunk's count.
+__global__ __launch_bounds__(kThreads) void k_rs_rank(const uint32_t* __restrict__ ent,
+                                                      const uint8_t* __restrict__ hit,
+                                                      const uint16_t* __restrict__ starts, int B,
+                                                      uint16_t* __restrict__ lrank, uint32_t* __restrict__ cnt) {
+  __shared__ __align__(16) uint8_t flag[kRsChunk];
+  __shared__ __align__(16) uint16_t rk[kRsChunk];
+  __shared__ uint32_t s_wave[kThreads / 64 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const uint64_t base = (uint64_t)blockIdx.x * kRsChunk;
+  const uint32_t valid = starts[(size_t)blockIdx.x * (B + 1) + B];
+  ((uint32_t*)flag)[tid * 4 + 0] = 0;
+  ((uint32_t*)flag)[tid * 4 + 1] = 0;
+  ((uint32_t*)flag)[tid * 4 + 2] = 0;
+  ((uint32_t*)flag)[tid * 4 + 3] = 0;
+  __syncthreads();
+  for (uint32_t q = tid; q < valid; q += kThreads)
+    if (hit[base + q]) flag[ent[base + q] & (kRsChunk - 1)] = 1;
+  __syncthreads();
+  // thread t ranks darts 16 t .. 16 t + 15
+  const u32x4 f4 = ((const u32x4*)flag)[tid];
+  const uint32_t fw[4] = {f4.x, f4.y, f4.z, f4.w};
+  uint32_t n = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) n += __popc(fw[i]);  // one flag byte holds 0 or 1
+  uint32_t incl = n;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) s_wave[wv] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int q = 0; q < kThreads / 64; q++) {
+      const uint32_t t = s_wave[q];
+      s_wave[q] = run;
+      run += t;
+    }
+    s_wave[kThreads / 64] = run;
+  }
+  __syncthreads();
+  uint32_t rr = s_wave[wv] + incl - n;
+#pragma unroll
+  for (int i = 0; i < kDartsPerThread; i++) {
+    const bool h = (fw[i >> 2] >> (8 * (i & 3))) & 1u;
+    rk[tid * kDartsPerThread + i] = h ? (uint16_t)rr : kNoRank;
+    rr += h;
+  }
+  __syncthreads();
+  for (uint32_t q = tid; q < valid; q += kThreads) lrank[base + q] = rk[ent[base + q] & (kRsChunk - 1)];
+  if (tid == 0) cnt[blockIdx.x] = s_wave[kThreads / 64];
+}
+
+// Items of the region's slots in slot order (first[]), the filled slots per tile.
+__global__ __launch_bounds__(kThreads) void k_rs_place(const uint32_t* __restrict__ ent,
+                                                       const uint16_t* __restrict__ lrank,
+                                                       const uint16_t* __restrict__ offT,
+                                                       const uint32_t* __restrict__ coff, int64_t C, int B,
+                                                       uint32_t M, uint32_t N, uint32_t* __restrict__ first,
+                                                       uint32_t* __restrict__ tile_count) {
+  extern __shared__ uint32_t tbl[];  // [kRsRegion] item per slot
+  __shared__ uint32_t sh[kThreads / 64];
+  const int r = rs_region(blockIdx.x, B);
+  if (r >= B) return;
+  for (uint32_t i = threadIdx.x; i < kRsRegion; i += kThreads) tbl[i] = kEmpty;
+  __syncthreads();
+  constexpr int U = 4;
+  rs_walk<U>(offT, C, r, [&](int64_t c) { return c < C ? coff[c] : 0u; },
+             [&](const int64_t* pos, const uint32_t* cbase) {
+               uint32_t v[U];
+               uint16_t k[U];
+#pragma unroll
+               for (int q = 0; q < U; q++) {
+                 v[q] = pos[q] >= 0 ? ent[pos[q]] : 0u;
+                 k[q] = pos[q] >= 0 ? lrank[pos[q]] : kNoRank;
+               }
+#pragma unroll
+               for (int q = 0; q < U; q++)
+                 if (k[q] != kNoRank) {
+                   const uint32_t item = cbase[q] + k[q];
+                   if (item < N) tbl[v[q] >> kRsChunkBits] = item;
+                 }
+             });
+  __syncthreads();  // every item is in place
+  const uint64_t s0 = (uint64_t)r * kRsRegion;
+  uint32_t n0 = 0, n1 = 0;
+  for (uint32_t i = threadIdx.x; i < kRsRegion; i += kThreads) {
+    const uint64_t sl = s0 + i;
+    if (sl < M) {
+      const uint32_t v = tbl[i];
+      first[sl] = v;
+      if (v != kEmpty) {
+        if (i < (uint32_t)kTile) n0++; else n1++;
+      }
+    }
+  }
+  const uint32_t t0 = block_sum(n0, sh);
+  const uint32_t t1 = block_sum(n1, sh);
+  const uint64_t nts = ((uint64_t)M + kTile - 1) / kTile;
+  if (threadIdx.x == 0) {
+    tile_count[2 * r] = t0;
+    if ((uint64_t)2 * r + 1 < nts) tile_count[2 * r + 1] = t1;
+  }
+}
+
 // perm holds 0..N-1 once each: seen[] marks values; any duplicate or value out
 // of range sets *bad.
 __global__ __launch_bounds__(256) void k_check_perm(const int64_t* __restrict__ perm, uint64_t N,
@@ -329,6 +650,61 @@ static uint64_t randperm_darts(uint64_t N) {
   return (uint64_t)(1.3862944 * (double)N + 8.0 * sqrt((double)N)) + 4096;
 }
 
+static uint64_t up256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
+
+// The darts resolved by slot region (M <= kRsRegion * kRsMaxRegions).
+static int randperm_by_region(int64_t* perm, uint64_t N, uint64_t M, uint64_t x0, Device* d, hipStream_t s,
+                              int64_t* darts_used) {
+  using namespace rp;
+  const int B = (int)((M + kRsRegion - 1) / kRsRegion);
+  const unsigned grid_r = (unsigned)(8 * ((B + 7) / 8));
+  const size_t lds_sort = (size_t)(((B + 3) & ~3) + kRsChunk) * 4;
+  const size_t lds_region = (size_t)kRsRegion * 4;
+  for (uint64_t D = randperm_darts(N);; D *= 2) {
+    if (D >= ((uint64_t)1 << 31)) return (int)hipErrorInvalidValue;
+    const uint64_t C = (D + kRsChunk - 1) / kRsChunk, nts = (M + kTile - 1) / kTile;
+    const uint64_t nt = C > nts ? C : nts;
+    const uint64_t b_ent = up256(C * kRsChunk * 4), b_st = up256(C * (B + 1) * 2), b_hit = up256(C * kRsChunk),
+                   b_rank = up256(C * kRsChunk * 2), b_first = up256(M * 4), b_cnt = up256(nt * 4);
+    char* ws = (char*)workspace(d, b_ent + 2 * b_st + b_hit + b_rank + b_first + b_cnt + up256((nt + 1) * 4), s);
+    if (!ws) return (int)hipErrorOutOfMemory;
+    uint32_t* ent = (uint32_t*)ws;
+    uint16_t* starts = (uint16_t*)(ws + b_ent);
+    uint16_t* offT = (uint16_t*)(ws + b_ent + b_st);
+    uint8_t* hit = (uint8_t*)(ws + b_ent + 2 * b_st);
+    uint16_t* lrank = (uint16_t*)(ws + b_ent + 2 * b_st + b_hit);
+    uint32_t* first = (uint32_t*)(ws + b_ent + 2 * b_st + b_hit + b_rank);
+    uint32_t* cnt = (uint32_t*)(ws + b_ent + 2 * b_st + b_hit + b_rank + b_first);
+    uint32_t* off = (uint32_t*)(ws + b_ent + 2 * b_st + b_hit + b_rank + b_first + b_cnt);
+    allow_lds(k_rs_sort, lds_sort);
+    hipLaunchKernelGGL(k_rs_sort, dim3((unsigned)C), dim3(kThreads), lds_sort, s, x0, D, (uint32_t)M, B, ent,
+                       starts);
+    int rc = launch_starts_transpose(starts, offT, B + 1, (int64_t)C, s);
+    if (rc) return rc;
+    allow_lds(k_rs_first, lds_region);
+    hipLaunchKernelGGL(k_rs_first, dim3(grid_r), dim3(kThreads), lds_region, s, (const uint32_t*)ent,
+                       (const uint16_t*)offT, (int64_t)C, B, hit);
+    hipLaunchKernelGGL(k_rs_rank, dim3((unsigned)C), dim3(kThreads), 0, s, (const uint32_t*)ent,
+                       (const uint8_t*)hit, (const uint16_t*)starts, B, lrank, cnt);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, s, (const uint32_t*)cnt, (uint32_t)C, off);
+    uint32_t hits = 0;
+    rc = (int)hipMemcpyAsync(&hits, off + C, 4, hipMemcpyDeviceToHost, s);
+    if (!rc) rc = (int)hipStreamSynchronize(s);
+    if (rc) return rc;
+    if (hits < N) continue;  // too few darts: again with twice as many
+    // off: the chunks' offsets, read by k_rs_place; cnt then holds tile counts
+    allow_lds(k_rs_place, lds_region);
+    hipLaunchKernelGGL(k_rs_place, dim3(grid_r), dim3(kThreads), lds_region, s, (const uint32_t*)ent,
+                       (const uint16_t*)lrank, (const uint16_t*)offT, (const uint32_t*)off, (int64_t)C, B,
+                       (uint32_t)M, (uint32_t)N, first, cnt);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, s, (const uint32_t*)cnt, (uint32_t)nts, off);
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)nts), dim3(kThreads), 0, s, (const uint32_t*)first, M,
+                       (const uint32_t*)off, perm);
+    if (darts_used) *darts_used = (int64_t)D;
+    return (int)hipGetLastError();
+  }
+}
+
 int launch_randperm(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream_t s,
                     int64_t* darts_used) {
   using namespace rp;
@@ -336,6 +712,8 @@ int launch_randperm(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream
   const uint64_t M = 2 * (uint64_t)N;
   // srand48(seed) when seed != 0, else drand48's initial state
   const uint64_t x0 = seed != 0 ? ((uint64_t)(uint32_t)seed << 16) | 0x330EULL : 0x1234ABCD330EULL;
+  if (M <= (uint64_t)kRsRegion * kRsMaxRegions) return randperm_by_region(perm, (uint64_t)N, M, x0, d, s, darts_used);
+  // larger: first darts by device atomic min over all slots
   for (uint64_t D = randperm_darts((uint64_t)N);; D *= 2) {
     if (D >= ((uint64_t)1 << 31)) return (int)hipErrorInvalidValue;
     const uint64_t ntd = (D + kTile - 1) / kTile, nts = (M + kTile - 1) / kTile;

@@ -206,8 +206,23 @@ def test_randperm_matches_oracle(dev, orc, N, seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("N,seed", [(1 << 26, 101892), ((1 << 26) - 12345, 5)])
+def test_randperm_by_region_largest_matches_oracle(dev, orc, N, seed):
+    """The largest permutations resolved by slot region (up to 2^27 slots in
+    4096 regions of 2^15; ~9.3e7 darts in 5679 chunks, so each region's walk
+    spans several segments of chunks), the second with a partial last region,
+    element for element."""
+    import torch
+    perm = torch.empty(N, dtype=torch.int64, device=dev)
+    B.randperm(perm, seed)
+    want, _ = orc.randperm_agi(N, seed)
+    assert np.array_equal(perm.cpu().numpy(), want)
+
+
+@pytest.mark.gpu
 def test_randperm_large_matches_oracle(dev, orc):
-    """2^27 elements (2^28 slots, ~1.9e8 darts), element for element."""
+    """2^27 elements (2^28 slots, ~1.9e8 darts: first darts by device atomic
+    min), element for element."""
     import torch
     N = 1 << 27
     perm = torch.empty(N, dtype=torch.int64, device=dev)
