@@ -6,7 +6,10 @@
 // thrown in turn, and the i-th dart that finds its slot empty places item i;
 // the permutation lists the filled slots in order.  The outcome has a parallel
 // statement: dart k hits iff no earlier dart chose its slot, and the item it
-// places is the number of hits before it (placing stops at N items).  So:
+// places is the number of hits before it (placing stops at N items).  From one
+// 2^15-slot region per CU up to 2^27 slots the darts are sorted by slot region
+// and resolved in LDS (the k_rs_* kernels below, no device atomics and no
+// random global accesses); otherwise over the whole slot array:
 //   k_darts    each dart's slot, from the drand48 sequence (jump-ahead per thread)
 //   k_first    first[slot] = smallest dart index choosing it (device atomic min)
 //   k_hits     darts that are their slot's first, flagged (bit 31), counted per tile
@@ -235,6 +238,7 @@ constexpr int kRsRegionBits = 15;
 constexpr uint32_t kRsRegion = 1u << kRsRegionBits;  // slots per region (2 tiles)
 constexpr int kRsMaxRegions = 4096;
 constexpr uint16_t kNoRank = 0xFFFF;
+constexpr uint32_t kRsFirst = 0x80000000u;  // entry flag: its dart is its slot's first
 static_assert(kRsChunk == (uint32_t)kTile && kRsRegion == 2 * (uint32_t)kTile, "tiles and regions");
 
 // Region served by workgroup b: consecutive regions on one XCD (workgroup b
@@ -390,10 +394,11 @@ __device__ inline void rs_walk(const uint16_t* __restrict__ offT, int64_t C, int
   }
 }
 
-// first dart of each slot of the region, then a flag per entry: its dart is first
-__global__ __launch_bounds__(kThreads) void k_rs_first(const uint32_t* __restrict__ ent,
-                                                       const uint16_t* __restrict__ offT, int64_t C, int B,
-                                                       uint8_t* __restrict__ hit) {
+// First dart of each slot of the region; then each entry whose dart is its
+// slot's first is flagged in place (bit 31, above the entry's 29 bits), so the
+// flags land in the entry lines the walk has just read.
+__global__ __launch_bounds__(kThreads) void k_rs_first(uint32_t* __restrict__ ent,
+                                                       const uint16_t* __restrict__ offT, int64_t C, int B) {
   extern __shared__ uint32_t tbl[];  // [kRsRegion] smallest dart per slot
   const int r = rs_region(blockIdx.x, B);
   if (r >= B) return;
@@ -421,7 +426,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_first(const uint32_t* __restric
     for (int q = 0; q < U; q++)
       if (pos[q] >= 0) {
         const uint32_t k = (uint32_t)(pos[q] & ~(int64_t)(kRsChunk - 1)) | (v[q] & (kRsChunk - 1));
-        hit[pos[q]] = tbl[v[q] >> kRsChunkBits] == k ? 1 : 0;
+        if (tbl[v[q] >> kRsChunkBits] == k) ent[pos[q]] = v[q] | kRsFirst;
       }
   });
 }
@@ -429,7 +434,6 @@ __global__ __launch_bounds__(kThreads) void k_rs_first(const uint32_t* __restric
 // Ranks of the chunk's first darts in dart order: per entry (kNoRank for the
 // others) and the chunk's count.
 __global__ __launch_bounds__(kThreads) void k_rs_rank(const uint32_t* __restrict__ ent,
-                                                      const uint8_t* __restrict__ hit,
                                                       const uint16_t* __restrict__ starts, int B,
                                                       uint16_t* __restrict__ lrank, uint32_t* __restrict__ cnt) {
   __shared__ __align__(16) uint8_t flag[kRsChunk];
@@ -444,7 +448,10 @@ __global__ __launch_bounds__(kThreads) void k_rs_rank(const uint32_t* __restrict
   ((uint32_t*)flag)[tid * 4 + 3] = 0;
   __syncthreads();
   for (uint32_t q = tid; q < valid; q += kThreads)
-    if (hit[base + q]) flag[ent[base + q] & (kRsChunk - 1)] = 1;
+  {
+    const uint32_t v = ent[base + q];
+    if (v & kRsFirst) flag[v & (kRsChunk - 1)] = 1;
+  }
   __syncthreads();
   // thread t ranks darts 16 t .. 16 t + 15
   const u32x4 f4 = ((const u32x4*)flag)[tid];
@@ -509,7 +516,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_place(const uint32_t* __restric
                for (int q = 0; q < U; q++)
                  if (k[q] != kNoRank) {
                    const uint32_t item = cbase[q] + k[q];
-                   if (item < N) tbl[v[q] >> kRsChunkBits] = item;
+                   if (item < N) tbl[(v[q] >> kRsChunkBits) & (kRsRegion - 1)] = item;
                  }
              });
   __syncthreads();  // every item is in place
@@ -664,28 +671,27 @@ static int randperm_by_region(int64_t* perm, uint64_t N, uint64_t M, uint64_t x0
     if (D >= ((uint64_t)1 << 31)) return (int)hipErrorInvalidValue;
     const uint64_t C = (D + kRsChunk - 1) / kRsChunk, nts = (M + kTile - 1) / kTile;
     const uint64_t nt = C > nts ? C : nts;
-    const uint64_t b_ent = up256(C * kRsChunk * 4), b_st = up256(C * (B + 1) * 2), b_hit = up256(C * kRsChunk),
+    const uint64_t b_ent = up256(C * kRsChunk * 4), b_st = up256(C * (B + 1) * 2),
                    b_rank = up256(C * kRsChunk * 2), b_first = up256(M * 4), b_cnt = up256(nt * 4);
-    char* ws = (char*)workspace(d, b_ent + 2 * b_st + b_hit + b_rank + b_first + b_cnt + up256((nt + 1) * 4), s);
+    char* ws = (char*)workspace(d, b_ent + 2 * b_st + b_rank + b_first + b_cnt + up256((nt + 1) * 4), s);
     if (!ws) return (int)hipErrorOutOfMemory;
     uint32_t* ent = (uint32_t*)ws;
     uint16_t* starts = (uint16_t*)(ws + b_ent);
     uint16_t* offT = (uint16_t*)(ws + b_ent + b_st);
-    uint8_t* hit = (uint8_t*)(ws + b_ent + 2 * b_st);
-    uint16_t* lrank = (uint16_t*)(ws + b_ent + 2 * b_st + b_hit);
-    uint32_t* first = (uint32_t*)(ws + b_ent + 2 * b_st + b_hit + b_rank);
-    uint32_t* cnt = (uint32_t*)(ws + b_ent + 2 * b_st + b_hit + b_rank + b_first);
-    uint32_t* off = (uint32_t*)(ws + b_ent + 2 * b_st + b_hit + b_rank + b_first + b_cnt);
+    uint16_t* lrank = (uint16_t*)(ws + b_ent + 2 * b_st);
+    uint32_t* first = (uint32_t*)(ws + b_ent + 2 * b_st + b_rank);
+    uint32_t* cnt = (uint32_t*)(ws + b_ent + 2 * b_st + b_rank + b_first);
+    uint32_t* off = (uint32_t*)(ws + b_ent + 2 * b_st + b_rank + b_first + b_cnt);
     allow_lds(k_rs_sort, lds_sort);
     hipLaunchKernelGGL(k_rs_sort, dim3((unsigned)C), dim3(kThreads), lds_sort, s, x0, D, (uint32_t)M, B, ent,
                        starts);
     int rc = launch_starts_transpose(starts, offT, B + 1, (int64_t)C, s);
     if (rc) return rc;
     allow_lds(k_rs_first, lds_region);
-    hipLaunchKernelGGL(k_rs_first, dim3(grid_r), dim3(kThreads), lds_region, s, (const uint32_t*)ent,
-                       (const uint16_t*)offT, (int64_t)C, B, hit);
+    hipLaunchKernelGGL(k_rs_first, dim3(grid_r), dim3(kThreads), lds_region, s, ent,
+                       (const uint16_t*)offT, (int64_t)C, B);
     hipLaunchKernelGGL(k_rs_rank, dim3((unsigned)C), dim3(kThreads), 0, s, (const uint32_t*)ent,
-                       (const uint8_t*)hit, (const uint16_t*)starts, B, lrank, cnt);
+                       (const uint16_t*)starts, B, lrank, cnt);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, s, (const uint32_t*)cnt, (uint32_t)C, off);
     uint32_t hits = 0;
     rc = (int)hipMemcpyAsync(&hits, off + C, 4, hipMemcpyDeviceToHost, s);
@@ -712,8 +718,14 @@ int launch_randperm(int64_t* perm, int64_t N, int64_t seed, Device* d, hipStream
   const uint64_t M = 2 * (uint64_t)N;
   // srand48(seed) when seed != 0, else drand48's initial state
   const uint64_t x0 = seed != 0 ? ((uint64_t)(uint32_t)seed << 16) | 0x330EULL : 0x1234ABCD330EULL;
-  if (M <= (uint64_t)kRsRegion * kRsMaxRegions) return randperm_by_region(perm, (uint64_t)N, M, x0, d, s, darts_used);
-  // larger: first darts by device atomic min over all slots
+  // By region when there are regions for every CU (a region's walk takes the
+  // same time at any size, so with fewer regions than CUs the atomics win:
+  // measured on MI355X, 2^20 elements 0.17 against 0.41 ms, 2^21 0.26 against
+  // 0.27, 2^22 0.46 against 0.21) and at most kRsMaxRegions of them.
+  const uint64_t nreg = (M + kRsRegion - 1) / kRsRegion;
+  if (nreg <= (uint64_t)kRsMaxRegions && nreg >= (uint64_t)d->cus)
+    return randperm_by_region(perm, (uint64_t)N, M, x0, d, s, darts_used);
+  // otherwise first darts by device atomic min over all slots
   for (uint64_t D = randperm_darts((uint64_t)N);; D *= 2) {
     if (D >= ((uint64_t)1 << 31)) return (int)hipErrorInvalidValue;
     const uint64_t ntd = (D + kTile - 1) / kTile, nts = (M + kTile - 1) / kTile;

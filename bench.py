@@ -412,10 +412,11 @@ def bench_randperm(B, torch, dev, stream, N=1 << 26, reps=5, seed=101892):
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
     t = median(ts)
-    darts = int(1.3862944 * N + 8.0 * N ** 0.5) + 4096  # one device atomic min each
+    darts = int(1.3862944 * N + 8.0 * N ** 0.5) + 4096  # darts generated
     return {"n": N, "seed": seed, "median_s": t, "Gelems_per_s": N / t / 1e9, "darts": darts,
-            "note": "wall time of one call (darts, first-dart table, hit ranks, placement, "
-                    "compaction; one 4-byte readback of the hit count)"}
+            "note": "wall time of one call (darts sorted by 2^15-slot region, each slot's first dart "
+                    "and the hits' ranks resolved in LDS, items placed region by region, compaction; "
+                    "one 4-byte readback of the hit count)"}
 
 
 def cpu_randperm(N=1 << 24, seed=101892):
@@ -526,6 +527,7 @@ WORKLOADS = {
     "small": "the planned histogram of 2^24 uniform updates into 2^20 int64 bins",
     "batch": "the held-out mix of 4096 instances (seed 5), learned choices, launched three times",
     "transpose": "three transposes of the Erdos-Renyi matrix of 2^24 rows (p = 18 / n, both triangles)",
+    "randperm": "three random permutations of 2^26 elements (rand_permp_agi, seed 101892)",
 }
 
 
@@ -571,6 +573,12 @@ def run_workloads(which) -> int:
             toff, tnz = B.spmat_transpose(nr, off, nz)
         torch.cuda.synchronize()
         del off, nz, toff, tnz
+    if "randperm" in which:
+        perm = torch.empty(1 << 26, dtype=torch.int64, device=dev)
+        for _ in range(3):
+            B.randperm(perm, 101892)
+        torch.cuda.synchronize()
+        del perm
     if "batch" in which:
         del idx
         from bale_old_amd import mix as MX, selector as S

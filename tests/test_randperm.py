@@ -206,11 +206,12 @@ def test_randperm_matches_oracle(dev, orc, N, seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,seed", [(1 << 26, 101892), ((1 << 26) - 12345, 5)])
-def test_randperm_by_region_largest_matches_oracle(dev, orc, N, seed):
-    """The largest permutations resolved by slot region (up to 2^27 slots in
-    4096 regions of 2^15; ~9.3e7 darts in 5679 chunks, so each region's walk
-    spans several segments of chunks), the second with a partial last region,
+@pytest.mark.parametrize("N,seed", [(1 << 26, 101892), ((1 << 26) - 12345, 5), ((1 << 22) - 1, 3),
+                                    ((1 << 22) + 7, -3)])
+def test_randperm_by_region_matches_oracle(dev, orc, N, seed):
+    """Permutations resolved by slot region, which runs from one 2^15-slot
+    region per CU (2^22 - 1 elements: 256 regions, the last one short) to
+    2^27 slots (2^26 elements: 4096 regions, ~9.3e7 darts in 5679 chunks),
     element for element."""
     import torch
     perm = torch.empty(N, dtype=torch.int64, device=dev)
