@@ -217,15 +217,15 @@ __global__ __launch_bounds__(kThreads) void k_compact(const uint32_t* __restrict
 }
 
 // ---- the same darts resolved by slot region, in LDS -----------------------
-// For up to kRsMaxRegions regions of 2^15 slots (M <= 2^27) no dart meets a
-// device atomic or a random global access:
+// With one region of 2^15 slots per CU up to kRsMaxRegions of them (M <= 2^27)
+// no dart meets a device atomic or a random global access:
 //   k_rs_sort   one workgroup per chunk of 2^14 darts generates them and writes
 //               them grouped by region as u32 entries (slot in region << 14 |
 //               dart in chunk), with the chunk's run starts (one u16 row)
 //   (the run starts transposed to one row per region)
 //   k_rs_first  one workgroup per region walks its runs chunk by chunk: an LDS
 //               atomic min per slot gives each slot's first dart, a second walk
-//               flags the entries whose dart is that first one (a byte each)
+//               flags the entries whose dart is that first one (bit 31)
 //   k_rs_rank   one workgroup per chunk: the flagged darts' ranks in dart order
 //               within the chunk (u16 per entry) and the chunk's count
 //   k_rs_place  one workgroup per region: each slot's item (the chunk's offset
@@ -447,8 +447,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_rank(const uint32_t* __restrict
   ((uint32_t*)flag)[tid * 4 + 2] = 0;
   ((uint32_t*)flag)[tid * 4 + 3] = 0;
   __syncthreads();
-  for (uint32_t q = tid; q < valid; q += kThreads)
-  {
+  for (uint32_t q = tid; q < valid; q += kThreads) {
     const uint32_t v = ent[base + q];
     if (v & kRsFirst) flag[v & (kRsChunk - 1)] = 1;
   }
