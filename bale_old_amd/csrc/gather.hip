@@ -6,11 +6,14 @@
 //           as 16-byte non-temporal streams so they do not evict table lines.
 //           The AGI model (apps/ig_src/ig_agi.upc:61-64).
 //   BUCKET  requests sorted by table region within chunks, each region served
-//           from one XCD's L2, replies put back in order through LDS (below) --
+//           from one XCD's L2 (or, for regions of at most 2^14 entries, from a
+//           workgroup's LDS copy of it), replies put back in order through LDS
+//           (below) --
 //           the request/reply exchange of apps/ig_src/ig_exstack.upc:69-94 and
 //           ig_conveyor.upc:78-101.
 #include "bale_internal.h"
 #include "partition.h"
+#include "runwalk.h"
 
 namespace bale {
 
@@ -322,6 +325,38 @@ __global__ __launch_bounds__(kGRegThreads) void k_gregion(
   }
 }
 
+// Regions of at most 2^14 entries (the plan's choice for tables of up to 2^25
+// entries) are served from LDS instead: one workgroup per region copies its slice of the
+// table (at most 128 KiB) into LDS with coalesced loads, then walks the
+// region's runs over all chunks and answers each request with a random LDS
+// read, which the CU serves many times faster than random L2 reads.  The
+// reply still goes to the entry's position.
+constexpr int kGLdsMaxBits = 14;
+__global__ __launch_bounds__(1024) void k_gregion_lds(const uint32_t* __restrict__ sorted,
+                                                      const uint16_t* __restrict__ offT, int B, int64_t C,
+                                                      int w, const int64_t* __restrict__ table,
+                                                      int64_t tlen, int64_t* __restrict__ reply,
+                                                      const uint32_t* __restrict__ flags) {
+  if (flags && gather_skewed(flags)) return;
+  extern __shared__ int64_t tbl[];  // [1 << w]
+  const int r = region_of_block(blockIdx.x, B);
+  if (r >= B) return;
+  const int64_t t0 = (int64_t)r << w;
+  const int64_t tn = min((int64_t)1 << w, tlen - t0);
+  for (int64_t i = threadIdx.x; i < tn; i += 1024) tbl[i] = table[t0 + i];
+  __syncthreads();
+  constexpr int Q = 4;
+  auto none = [](int64_t) { return 0u; };
+  region_walk<1024, kGSlotBits, Q>(offT, C, r, none, [&](const int64_t* pos, const uint32_t*) {
+    uint32_t o[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) o[q] = pos[q] >= 0 ? sorted[pos[q]] : 0u;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      if (pos[q] >= 0) reply[pos[q]] = tbl[o[q] >> kGSlotBits];
+  });
+}
+
 // One workgroup per chunk: tgt of the chunk from its entries and replies.
 __global__ __launch_bounds__(1024) void k_gunpermute(const uint32_t* __restrict__ sorted,
                                                      const int64_t* __restrict__ reply,
@@ -418,9 +453,11 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
                          hipStream_t s) {
   if (n <= 0) return 0;
   if (n >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;  // caller splits
-  // regions of 2^18 entries (2 MiB, half an XCD's L2) unless set; an entry
-  // holds the offset in the region beside the slot in the chunk
+  // regions of 2^tile_log2 entries, 2^18 (2 MiB, half an XCD's L2) unless
+  // set; those of at most 2^14 entries are served from LDS; an entry holds the
+  // offset in the region beside the slot in the chunk
   int w = c.tile_log2 > 0 ? std::min(c.tile_log2, kGRegionMaxBits) : 18;
+  const bool lds_regions = w <= kGLdsMaxBits;
   int64_t B = ceil_div(table_len, (int64_t)1 << w);
   if (B > kMaxRegions)
     return bad_config("BUCKET gather: a table of %lld entries needs more than %d regions of at most "
@@ -465,17 +502,25 @@ int launch_gather_bucket(int64_t* tgt, const int64_t* idx, int64_t n, const int6
   hipLaunchKernelGGL(k_gtranspose, dim3((unsigned)ceil_div(B + 1, 64), (unsigned)ceil_div(C, 64)),
                      dim3(256), 0, s, (const uint16_t*)starts, offT, (int)(B + 1), C,
                      (const uint32_t*)flags);
-  // segments of about 2^task_log2 requests (a chunk holds kGChunk / B of a
-  // region's on average), 2^11 by default: measured on MI355X at 2^28 requests
-  // into a 2^28-entry table against 2^10 and 2^12 (4.64 against 4.89 and
-  // 4.85 ms), and 4 workgroups per CU against 2 and 8 (5.25, 4.73 ms)
-  const int seg_log2 = c.task_log2 > 0 ? c.task_log2 : 11;
-  const int seg_chunks =
-      (int)std::max<int64_t>(16, std::min<int64_t>(kGSegChunks, ((int64_t)B << seg_log2) >> kGSlotBits));
-  int gr = c.groups_per_cu > 0 ? c.groups_per_cu : 4;
-  hipLaunchKernelGGL(k_gregion, dim3((unsigned)(d->cus * gr)), dim3(kGRegThreads), 0, s,
-                     (const uint32_t*)sorted, (const uint16_t*)offT, (int)B, C, w, seg_chunks, table,
-                     reply, heads, (const uint32_t*)flags);
+  if (lds_regions) {
+    const size_t lds_t = (size_t)8 << w;
+    allow_lds(k_gregion_lds, lds_t);
+    hipLaunchKernelGGL(k_gregion_lds, dim3((unsigned)(8 * ceil_div(B, 8))), dim3(1024), lds_t, s,
+                       (const uint32_t*)sorted, (const uint16_t*)offT, (int)B, C, w, table, table_len, reply,
+                       (const uint32_t*)flags);
+  } else {
+    // segments of about 2^task_log2 requests (a chunk holds kGChunk / B of a
+    // region's on average), 2^11 by default: measured on MI355X at 2^28
+    // requests into a 2^28-entry table against 2^10 and 2^12 (4.64 against
+    // 4.89 and 4.85 ms), and 4 workgroups per CU against 2 and 8 (5.25, 4.73 ms)
+    const int seg_log2 = c.task_log2 > 0 ? c.task_log2 : 11;
+    const int seg_chunks =
+        (int)std::max<int64_t>(16, std::min<int64_t>(kGSegChunks, ((int64_t)B << seg_log2) >> kGSlotBits));
+    int gr = c.groups_per_cu > 0 ? c.groups_per_cu : 4;
+    hipLaunchKernelGGL(k_gregion, dim3((unsigned)(d->cus * gr)), dim3(kGRegThreads), 0, s,
+                       (const uint32_t*)sorted, (const uint16_t*)offT, (int)B, C, w, seg_chunks, table,
+                       reply, heads, (const uint32_t*)flags);
+  }
   const size_t lds_un = (size_t)kGChunk * 8;
   allow_lds(k_gunpermute, lds_un);
   hipLaunchKernelGGL(k_gunpermute, dim3((unsigned)C), dim3(1024), lds_un, s, (const uint32_t*)sorted,

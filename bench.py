@@ -528,6 +528,8 @@ WORKLOADS = {
     "batch": "the held-out mix of 4096 instances (seed 5), learned choices, launched three times",
     "transpose": "three transposes of the Erdos-Renyi matrix of 2^24 rows (p = 18 / n, both triangles)",
     "randperm": "three random permutations of 2^26 elements (rand_permp_agi, seed 101892)",
+    "permute": "three permutes of the Erdos-Renyi matrix of 2^24 rows (row and column permutations "
+               "from rand_permp, seeds 1230 and 45)",
 }
 
 
@@ -573,6 +575,17 @@ def run_workloads(which) -> int:
             toff, tnz = B.spmat_transpose(nr, off, nz)
         torch.cuda.synchronize()
         del off, nz, toff, tnz
+    if "permute" in which:
+        nr = 1 << 24
+        off, nz = B.spmat_gen_er(nr, 18 / nr, 0, 3, seed=101894)
+        rpi = torch.empty(nr, dtype=torch.int64, device=dev)
+        cpi = torch.empty(nr, dtype=torch.int64, device=dev)
+        B.randperm(rpi, 1230)
+        B.randperm(cpi, 45)
+        for _ in range(3):
+            poff, pnz = B.spmat_permute(nr, off, nz, rpi, cpi)
+        torch.cuda.synchronize()
+        del off, nz, rpi, cpi, poff, pnz
     if "randperm" in which:
         perm = torch.empty(1 << 26, dtype=torch.int64, device=dev)
         for _ in range(3):

@@ -288,6 +288,30 @@ def test_gather_adaptive_plan(dev, orc, dist, s, run_mean):
         assert np.array_equal(to_np(tgt), want), packed
 
 
+@pytest.mark.parametrize("tlen,want_w", [((1 << 22) + 3, 13), ((1 << 24) - 1, 13), ((1 << 25) + 7, 14),
+                                         ((1 << 26) + 1, 18)])
+def test_gather_plan_regions(dev, orc, tlen, want_w):
+    """The planned BUCKET gather serves tables of up to 2^25 entries from LDS
+    regions of 2^13-2^14 entries (one workgroup per region, its slice of the
+    table copied in), larger ones from 2^18-entry L2 regions; ragged tables end
+    in a short region.  Plain and packed requests equal the oracle's ig_agi."""
+    n = max(1 << 24, (tlen + 1) // 2)  # the plan's BUCKET needs a request per two entries
+    plan = B.gather_plan(n, tlen)
+    assert plan.strategy == B.BUCKET and plan.tile_log2 == want_w
+    table = torch.empty(tlen, dtype=torch.int64, device=dev)
+    B.fill_splitmix(table)
+    idx = gen(dev, n, tlen, seed=tlen & 0xffff)
+    idx[-1] = tlen - 1  # the last entry of the short last region
+    want = orc.ig_agi(to_np(idx), orc.fill_splitmix(tlen))
+    pck = torch.empty_like(idx)
+    B.pack(pck, idx)
+    for cfg in (plan, B.config(B.BUCKET, tile_log2=want_w, hot=0)):
+        for packed, src in ((False, idx), (True, pck)):
+            tgt = torch.full((n,), 5, dtype=torch.int64, device=dev)
+            B.gather(tgt, src, table, packed=packed, cfg=cfg)
+            assert np.array_equal(to_np(tgt), want), (cfg, packed)
+
+
 def test_gather_bucket_errors_and_ragged(dev, orc):
     """BUCKET gather: out-of-range and foreign-PE requests are counted and
     their slots read 0; a ragged last chunk; regions narrower than default."""
