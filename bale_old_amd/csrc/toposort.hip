@@ -13,11 +13,14 @@
 // count) when the largest of each fit together, so one atomic add updates
 // both; otherwise in two arrays.
 //
-// Here a level runs as four launches:
+// Here a level runs as five launches:
 //   k_topo_pivot    one wave per queued row: claim the positions, then per
-//                   row s of the column: count and column sum decremented,
-//                   and key(k, j) = (k + 1) << 32 | j (j = place in the column)
-//                   kept as s's largest (device atomics)
+//                   row s of the column: count and column sum decremented
+//                   (a device atomic per nonzero)
+//   k_topo_mark     the same walk: for a row s whose count ended at 1, each
+//                   event's key(k, j) = (k + 1) << 32 | j (j = place in the
+//                   column) kept as s's largest (a device atomic per event on
+//                   such rows only)
 //   k_topo_count    a row s joins the next level iff its count ended at 1 and
 //                   the event is its last decrement of the level -- the one
 //                   that took the count from 2 to 1 in the sequential order --
@@ -229,7 +232,32 @@ __global__ __launch_bounds__(kWG) void k_topo_pivot(int64_t level, int64_t n,
         atomicSub(&w.rowcnt[s], 1);
         atomicAdd((unsigned long long*)&w.rowsum[s], (unsigned long long)(-c));
       }
-      atomicMax(&w.trig[s], (unsigned long long)trig_key(k, j - j0));
+    }
+  }
+}
+
+// One wave per queued row k of the level, after the pivots: every event (k, j)
+// on a row whose count ended the level at 1 offers its key to the row's
+// largest, so only those rows (about one per row joining the next level) take
+// the atomic; a row that does not end at 1 never joins, whatever its key.
+template <bool PK>
+__global__ __launch_bounds__(kCWG) void k_topo_mark(int64_t level, int64_t n,
+                                                    const int64_t* __restrict__ t_offset,
+                                                    const int64_t* __restrict__ t_nonzero, Ws w) {
+  const Level L = w.desc[level % kRing];
+  const int64_t m = L.qe - L.qs;
+  const int lane = __lane_id();
+  const int64_t nw = (int64_t)gridDim.x * (kCWG / 64);
+  for (int64_t e = (int64_t)blockIdx.x * (kCWG / 64) + (threadIdx.x >> 6); e < m; e += nw) {
+    const int64_t k = L.qs + e;
+    const int64_t c = w.qcol[k];
+    if (c < 0) continue;
+    const int64_t j0 = t_offset[c], j1 = t_offset[c + 1];
+    for (int64_t j = j0 + lane; j < j1; j += 64) {
+      const int64_t s = t_nonzero[j];
+      if ((uint64_t)s >= (uint64_t)n) continue;
+      const bool one = PK ? (w.cs[s] & ((1ull << w.cb) - 1)) == 1ull : w.rowcnt[s] == 1;
+      if (one) atomicMax(&w.trig[s], (unsigned long long)trig_key(k, j - j0));
     }
   }
 }
@@ -414,11 +442,13 @@ int launch_toposort(int64_t n, const int64_t* offset, const int64_t* nonzero, co
     for (int64_t l = L0; l < L0 + kBatch; l++) {
       if (pk) {
         hipLaunchKernelGGL(k_topo_pivot<true>, dim3(g), dim3(kWG), 0, s, l, n, t_offset, t_nonzero, rperm, cperm, w);
+        hipLaunchKernelGGL(k_topo_mark<true>, dim3(gc), dim3(kCWG), 0, s, l, n, t_offset, t_nonzero, w);
         hipLaunchKernelGGL(k_topo_count<true>, dim3(gc), dim3(kCWG), 0, s, l, n, t_offset, t_nonzero, w);
         hipLaunchKernelGGL(k_topo_scan, dim3(1), dim3(1024), 0, s, l, w);
         hipLaunchKernelGGL(k_topo_enqueue<true>, dim3(gc), dim3(kCWG), 0, s, l, n, t_offset, t_nonzero, w);
       } else {
         hipLaunchKernelGGL(k_topo_pivot<false>, dim3(g), dim3(kWG), 0, s, l, n, t_offset, t_nonzero, rperm, cperm, w);
+        hipLaunchKernelGGL(k_topo_mark<false>, dim3(gc), dim3(kCWG), 0, s, l, n, t_offset, t_nonzero, w);
         hipLaunchKernelGGL(k_topo_count<false>, dim3(gc), dim3(kCWG), 0, s, l, n, t_offset, t_nonzero, w);
         hipLaunchKernelGGL(k_topo_scan, dim3(1), dim3(1024), 0, s, l, w);
         hipLaunchKernelGGL(k_topo_enqueue<false>, dim3(gc), dim3(kCWG), 0, s, l, n, t_offset, t_nonzero, w);
