@@ -547,29 +547,35 @@ __global__ __launch_bounds__(256) void k_invert(const int64_t* __restrict__ rper
 
 // len[i] = length of row rperm[i] of A (0 and a range error for a row that
 // rperminv did not name)
+// Row i of the result: its length, and where its source row rperm[i] starts in
+// A (kept so the copy reads it in order instead of at random).
 __global__ __launch_bounds__(256) void k_perm_rowlen(const int64_t* __restrict__ offset,
                                                      const int64_t* __restrict__ rperm, int64_t n,
-                                                     int64_t* __restrict__ len, ErrWords* err) {
+                                                     int64_t* __restrict__ len, int64_t* __restrict__ src,
+                                                     ErrWords* err) {
   unsigned bad = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = rperm[i];
     if ((uint64_t)r < (uint64_t)n) {
-      len[i] = offset[r + 1] - offset[r];
+      const int64_t a = offset[r];
+      len[i] = offset[r + 1] - a;
+      src[i] = a;
     } else {
       len[i] = 0;
+      src[i] = 0;
       bad++;
     }
   }
   flush_errors(err, bad, 0);
 }
 
-// Row i of the result = row rperm[i] of A, columns mapped by cperminv (MAP),
-// or A's own columns (the map then runs as the library's gather).
+// Row i of the result = row rperm[i] of A (starting at srcs[i] in A), columns
+// mapped by cperminv (MAP), or A's own columns (the map then runs as the
+// library's gather).
 template <bool MAP>
-__global__ __launch_bounds__(kThreads) void k_perm_fill(const int64_t* __restrict__ offset,
+__global__ __launch_bounds__(kThreads) void k_perm_fill(const int64_t* __restrict__ srcs,
                                                         const int64_t* __restrict__ nonzero,
-                                                        const int64_t* __restrict__ rperm,
                                                         const int64_t* __restrict__ cperminv,
                                                         int64_t numrows, int64_t numcols,
                                                         const int64_t* __restrict__ p_offset,
@@ -580,10 +586,7 @@ __global__ __launch_bounds__(kThreads) void k_perm_fill(const int64_t* __restric
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerGroup;
   const int R = (int)std::min<int64_t>(kRowsPerGroup, numrows - r0);
   for (int i = threadIdx.x; i <= R; i += kThreads) so[i] = p_offset[r0 + i];
-  for (int i = threadIdx.x; i < R; i += kThreads) {
-    const int64_t r = rperm[r0 + i];
-    src[i] = (uint64_t)r < (uint64_t)numrows ? offset[r] : 0;
-  }
+  for (int i = threadIdx.x; i < R; i += kThreads) src[i] = srcs[r0 + i];
   __syncthreads();
   const int64_t p0 = so[0], p1 = std::min<int64_t>(so[R], nnz);
   unsigned bad = 0;
@@ -806,10 +809,11 @@ int launch_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const in
                          int64_t* p_offset, int64_t* p_nonzero, Device* d, hipStream_t s) {
   using namespace sm;
   const size_t rb = ((size_t)std::max<int64_t>(numrows, 1) * 8 + 255) & ~(size_t)255;
-  char* ws = (char*)workspace(d, rb + scan_tmp_bytes(numrows), s);
+  char* ws = (char*)workspace(d, 2 * rb + scan_tmp_bytes(numrows), s);
   if (!ws) return (int)hipErrorOutOfMemory;
   int64_t* rperm = (int64_t*)ws;
-  int64_t* stmp = (int64_t*)(ws + rb);
+  int64_t* srcs = (int64_t*)(ws + rb);  // where each result row's source row starts in A
+  int64_t* stmp = (int64_t*)(ws + 2 * rb);
   hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, rperm, numrows,
                      (int64_t)-1);
   hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, p_offset, (int64_t)1, (int64_t)0);
@@ -817,7 +821,7 @@ int launch_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const in
   hipLaunchKernelGGL(k_invert, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, rperminv, numrows,
                      rperm, stream_err(d, s));
   hipLaunchKernelGGL(k_perm_rowlen, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, offset,
-                     (const int64_t*)rperm, numrows, p_offset + 1, stream_err(d, s));
+                     (const int64_t*)rperm, numrows, p_offset + 1, srcs, stream_err(d, s));
   int rc = scan_incl(p_offset + 1, numrows, stmp, s);
   if (rc) return rc;
   // The column map is a gather from cperminv.  Large matrices copy the rows
@@ -829,11 +833,11 @@ int launch_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const in
   const bool by_gather = nnz >= ((int64_t)1 << 24) && numcols > 0;
   if (by_gather)
     hipLaunchKernelGGL(k_perm_fill<false>, dim3((unsigned)ceil_div(numrows, kRowsPerGroup)), dim3(kThreads),
-                       0, s, offset, nonzero, (const int64_t*)rperm, cperminv, numrows, numcols,
+                       0, s, (const int64_t*)srcs, nonzero, cperminv, numrows, numcols,
                        (const int64_t*)p_offset, nnz, p_nonzero, stream_err(d, s));
   else
     hipLaunchKernelGGL(k_perm_fill<true>, dim3((unsigned)ceil_div(numrows, kRowsPerGroup)), dim3(kThreads),
-                       0, s, offset, nonzero, (const int64_t*)rperm, cperminv, numrows, numcols,
+                       0, s, (const int64_t*)srcs, nonzero, cperminv, numrows, numcols,
                        (const int64_t*)p_offset, nnz, p_nonzero, stream_err(d, s));
   rc = (int)hipGetLastError();
   if (rc || !by_gather) return rc;
