@@ -545,10 +545,9 @@ __global__ __launch_bounds__(256) void k_invert(const int64_t* __restrict__ rper
   flush_errors(err, bad, 0);
 }
 
-// len[i] = length of row rperm[i] of A (0 and a range error for a row that
-// rperminv did not name)
-// Row i of the result: its length, and where its source row rperm[i] starts in
-// A (kept so the copy reads it in order instead of at random).
+// Row i of the result: the length of its source row rperm[i] of A and where
+// that row starts in A (kept so the copy reads the starts in order instead of
+// at random); 0 and a range error for a row that rperminv did not name.
 __global__ __launch_bounds__(256) void k_perm_rowlen(const int64_t* __restrict__ offset,
                                                      const int64_t* __restrict__ rperm, int64_t n,
                                                      int64_t* __restrict__ len, int64_t* __restrict__ src,
