@@ -532,39 +532,43 @@ __global__ __launch_bounds__(kThreads) void k_rx_count(const uint32_t* __restric
 }
 
 // ---- permutation ---------------------------------------------------------------------
-// rperm[rperminv[i]] = i; entries outside [0, n) are counted as range errors.
-__global__ __launch_bounds__(256) void k_invert(const int64_t* __restrict__ rperminv, int64_t n,
-                                                int64_t* __restrict__ rperm, ErrWords* err) {
+// A's row i becomes result row rperminv[i]: its length and start in A go there
+// as one 16-byte pair (A's offsets are read in order, and the result rows are
+// written at random only once); values outside [0, n) are range errors.
+__global__ __launch_bounds__(256) void k_perm_pairs(const int64_t* __restrict__ offset,
+                                                    const int64_t* __restrict__ rperminv, int64_t n,
+                                                    i64x2* __restrict__ pairs, ErrWords* err) {
   unsigned bad = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = rperminv[i];
-    if ((uint64_t)r < (uint64_t)n) rperm[r] = i;
-    else bad++;
+    if ((uint64_t)r < (uint64_t)n) {
+      const int64_t a = offset[i];
+      i64x2 v;
+      v.x = offset[i + 1] - a;
+      v.y = a;
+      pairs[r] = v;
+    } else {
+      bad++;
+    }
   }
   flush_errors(err, bad, 0);
 }
 
-// Row i of the result: the length of its source row rperm[i] of A and where
-// that row starts in A (kept so the copy reads the starts in order instead of
-// at random); 0 and a range error for a row that rperminv did not name.
-__global__ __launch_bounds__(256) void k_perm_rowlen(const int64_t* __restrict__ offset,
-                                                     const int64_t* __restrict__ rperm, int64_t n,
-                                                     int64_t* __restrict__ len, int64_t* __restrict__ src,
-                                                     ErrWords* err) {
+// The pairs split into the result's row lengths (scanned next) and the source
+// starts the copy reads in order; a result row that no row of A named keeps
+// the fill's start of -1: length 0 and a range error.
+__global__ __launch_bounds__(256) void k_perm_split(const i64x2* __restrict__ pairs, int64_t n,
+                                                    int64_t* __restrict__ len, int64_t* __restrict__ src,
+                                                    ErrWords* err) {
   unsigned bad = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = rperm[i];
-    if ((uint64_t)r < (uint64_t)n) {
-      const int64_t a = offset[r];
-      len[i] = offset[r + 1] - a;
-      src[i] = a;
-    } else {
-      len[i] = 0;
-      src[i] = 0;
-      bad++;
-    }
+    const i64x2 v = pairs[i];
+    const bool named = v.y >= 0;
+    len[i] = named ? v.x : 0;
+    src[i] = named ? v.y : 0;
+    bad += !named;
   }
   flush_errors(err, bad, 0);
 }
@@ -808,19 +812,19 @@ int launch_spmat_permute(int64_t numrows, int64_t numcols, int64_t nnz, const in
                          int64_t* p_offset, int64_t* p_nonzero, Device* d, hipStream_t s) {
   using namespace sm;
   const size_t rb = ((size_t)std::max<int64_t>(numrows, 1) * 8 + 255) & ~(size_t)255;
-  char* ws = (char*)workspace(d, 2 * rb + scan_tmp_bytes(numrows), s);
+  char* ws = (char*)workspace(d, 3 * rb + scan_tmp_bytes(numrows), s);
   if (!ws) return (int)hipErrorOutOfMemory;
-  int64_t* rperm = (int64_t*)ws;
-  int64_t* srcs = (int64_t*)(ws + rb);  // where each result row's source row starts in A
-  int64_t* stmp = (int64_t*)(ws + 2 * rb);
-  hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, rperm, numrows,
-                     (int64_t)-1);
+  i64x2* pairs = (i64x2*)ws;            // per result row: source length and start
+  int64_t* srcs = (int64_t*)(ws + 2 * rb);  // where each result row's source row starts in A
+  int64_t* stmp = (int64_t*)(ws + 3 * rb);
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(2 * numrows, 256, d)), dim3(256), 0, s, (int64_t*)pairs,
+                     2 * numrows, (int64_t)-1);
   hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, p_offset, (int64_t)1, (int64_t)0);
   if (numrows == 0) return (int)hipGetLastError();
-  hipLaunchKernelGGL(k_invert, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, rperminv, numrows,
-                     rperm, stream_err(d, s));
-  hipLaunchKernelGGL(k_perm_rowlen, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, offset,
-                     (const int64_t*)rperm, numrows, p_offset + 1, srcs, stream_err(d, s));
+  hipLaunchKernelGGL(k_perm_pairs, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, offset, rperminv,
+                     numrows, pairs, stream_err(d, s));
+  hipLaunchKernelGGL(k_perm_split, dim3(grid_for(numrows, 256, d)), dim3(256), 0, s, (const i64x2*)pairs,
+                     numrows, p_offset + 1, srcs, stream_err(d, s));
   int rc = scan_incl(p_offset + 1, numrows, stmp, s);
   if (rc) return rc;
   // The column map is a gather from cperminv.  Large matrices copy the rows
