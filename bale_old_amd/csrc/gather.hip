@@ -97,8 +97,8 @@ __global__ __launch_bounds__(1024) void k_gather_direct(int64_t* __restrict__ tg
 //   unpermute  one workgroup per chunk reads its entries and replies (two
 //              contiguous blocks) and places each reply at its slot in an LDS
 //              copy of the chunk's tgt, then writes it out as one stream.
-// Measured on MI355X at 2^28 requests into a 2^28-entry table: sort 0.73 ms,
-// transpose 0.03, region 3.0, unpermute 0.9.  Writing the replies region by
+// Measured on MI355X at 2^28 requests into a 2^28-entry table: sort 0.55 ms,
+// transpose 0.03, region 2.9, unpermute 0.9.  Writing the replies region by
 // region instead (one contiguous stream, positions from scans of the run
 // lengths) moved 0.55 ms from the region pass to the unpermute, which then
 // reads one short range per region: no gain, so the simpler layout stays.
@@ -112,7 +112,7 @@ constexpr int kGRegionMaxBits = 32 - kGSlotBits;
 constexpr int kXcds = 8;  // workgroup b runs on XCD b % 8 (round-robin dispatch)
 
 template <bool PACKED>
-__global__ __launch_bounds__(kGSortThreads) void k_gsort(const int64_t* __restrict__ idx, int64_t n,
+__global__ __launch_bounds__(kGSortThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_gsort(const int64_t* __restrict__ idx, int64_t n,
                                                         uint64_t tlen, int w, int B,
                                                         uint32_t* __restrict__ sorted,
                                                         uint16_t* __restrict__ starts,
@@ -129,34 +129,40 @@ __global__ __launch_bounds__(kGSortThreads) void k_gsort(const int64_t* __restri
   const uint32_t len = (uint32_t)min((int64_t)kGChunk, n - base);
   for (int r = tid; r < B; r += kGSortThreads) cnt[r] = 0;
   __syncthreads();
-  // element k of this thread: local position 2 * (p * 1024 + tid) + (k & 1), p = k / 2
-  uint32_t ent[kGPer], reg[kGPer], rank[kGPer];
+  // element k of this thread: local position 2 * (p * 1024 + tid) + (k & 1), p = k / 2;
+  // only its table index (below 2^30: at most 4096 regions of 2^18) is kept
+  // across the count, so the kernel fits 64 VGPRs and two workgroups per CU
+  // (0.55 ms at 2^28 requests, against 0.73 with each request's rank kept
+  // from the count and one workgroup per CU).  A request's place in its
+  // region's run comes from an atomic on the run's cursor, so a run holds its
+  // requests in no set order (each entry carries its slot).
+  uint32_t jj[kGPer];
   unsigned bad_r = 0, bad_p = 0;
-  const bool vec = (((uintptr_t)(idx + base)) & 15) == 0;
+  auto decode = [&](int k, uint32_t l, int64_t raw) {
+    uint64_t j = 0;
+    const bool ok = l < len && decode_index<PACKED>(raw, tlen, j, bad_r, bad_p);
+    jj[k] = ok ? (uint32_t)j : ~0u;
+  };
+  if (len == kGChunk && (((uintptr_t)(idx + base)) & 15) == 0) {
+    // a whole aligned chunk (every chunk but the last, in practice): 16-byte
+    // loads only
 #pragma unroll
-  for (int p = 0; p < kGPer / 2; p++) {
-    const uint32_t l0 = 2u * ((uint32_t)p * kGSortThreads + tid);
-    int64_t raw[2] = {0, 0};
-    if (vec && l0 + 1 < len) {
+    for (int p = 0; p < kGPer / 2; p++) {
+      const uint32_t l0 = 2u * ((uint32_t)p * kGSortThreads + tid);
       const i64x2 v = __builtin_nontemporal_load((const i64x2*)(idx + base + l0));
-      raw[0] = v.x;
-      raw[1] = v.y;
-    } else {
-      if (l0 < len) raw[0] = idx[base + l0];
-      if (l0 + 1 < len) raw[1] = idx[base + l0 + 1];
+      decode(2 * p, l0, v.x);
+      decode(2 * p + 1, l0 + 1, v.y);
     }
+  } else {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int k = 2 * p + h;
-      const uint32_t l = l0 + h;
-      uint64_t j = 0;
-      const bool ok = l < len && decode_index<PACKED>(raw[h], tlen, j, bad_r, bad_p);
-      reg[k] = ok ? (uint32_t)(j >> w) : ~0u;
-      ent[k] = ((uint32_t)(j & ((1ull << w) - 1)) << kGSlotBits) | l;
+    for (int k = 0; k < kGPer; k++) {
+      const uint32_t l = 2u * ((uint32_t)(k / 2) * kGSortThreads + tid) + (k & 1);
+      decode(k, l, l < len ? idx[base + l] : 0);
     }
   }
 #pragma unroll
-  for (int k = 0; k < kGPer; k++) rank[k] = reg[k] != ~0u ? atomicAdd(&cnt[reg[k]], 1u) : 0u;
+  for (int k = 0; k < kGPer; k++)
+    if (jj[k] != ~0u) atomicAdd(&cnt[jj[k] >> w], 1u);
   __syncthreads();
   // exclusive scan of cnt[0..B): each thread sums a run of up to 4 counters
   const int per = (B + kGSortThreads - 1) / kGSortThreads;
@@ -183,21 +189,25 @@ __global__ __launch_bounds__(kGSortThreads) void k_gsort(const int64_t* __restri
     s_valid = run;
   }
   __syncthreads();
+  uint16_t* row = starts + (size_t)blockIdx.x * (B + 1);
   uint32_t run = s_wave[wv] + incl - mine;
   for (int q = 0; q < per; q++)
     if (r0 + q < B) {
       const uint32_t c = cnt[r0 + q];
-      cnt[r0 + q] = run;
+      cnt[r0 + q] = run;  // the run's cursor
+      row[r0 + q] = (uint16_t)run;
       run += c;
     }
+  const uint32_t valid = s_valid;
+  if (tid == 0) row[B] = (uint16_t)valid;
   __syncthreads();
+  const uint32_t omask = (1u << w) - 1;
 #pragma unroll
   for (int k = 0; k < kGPer; k++)
-    if (reg[k] != ~0u) stage[cnt[reg[k]] + rank[k]] = ent[k];
-  const uint32_t valid = s_valid;
-  uint16_t* row = starts + (size_t)blockIdx.x * (B + 1);
-  for (int r = tid; r < B; r += kGSortThreads) row[r] = (uint16_t)cnt[r];
-  if (tid == 0) row[B] = (uint16_t)valid;
+    if (jj[k] != ~0u) {
+      const uint32_t l = 2u * ((uint32_t)(k / 2) * kGSortThreads + tid) + (k & 1);
+      stage[atomicAdd(&cnt[jj[k] >> w], 1u)] = ((jj[k] & omask) << kGSlotBits) | l;
+    }
   __syncthreads();
   uint32_t* out = sorted + base;
   for (uint32_t q = tid; q < valid / 4; q += kGSortThreads)
