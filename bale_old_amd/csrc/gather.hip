@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kGSortThreads) __attribute__((amdgpu_waves_per_eu(8
                                                         const uint32_t* __restrict__ flags) {
   if (flags && gather_skewed(flags)) return;  // the adaptive plan took DIRECT
   extern __shared__ uint32_t sh[];
-  uint32_t* cnt = sh;                     // [B] counts, then run starts
+  uint32_t* cnt = sh;                     // [B] counts, then run cursors
   uint32_t* stage = sh + ((B + 3) & ~3);  // [kGChunk] entries grouped by region
   __shared__ uint32_t s_wave[kGSortThreads / 64];
   __shared__ uint32_t s_valid;
