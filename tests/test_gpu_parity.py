@@ -312,6 +312,27 @@ def test_gather_plan_regions(dev, orc, tlen, want_w):
             assert np.array_equal(to_np(tgt), want), (cfg, packed)
 
 
+@pytest.mark.parametrize("tlen,w", [((1 << 25) - 5, 13), ((1 << 26) - 9, 14), (4095 * (1 << 11) + 1, 11)])
+def test_gather_most_regions(dev, orc, tlen, w):
+    """At the most regions a chunk's sort counts (4096, the last one short)
+    every chunk's run cursors fill the sort's LDS counter array: plain and
+    packed requests, with a ragged last chunk, equal the oracle's ig_agi."""
+    assert (tlen + (1 << w) - 1) >> w == 4096
+    n = (1 << 22) + 333
+    table = torch.empty(tlen, dtype=torch.int64, device=dev)
+    B.fill_splitmix(table)
+    idx = gen(dev, n, tlen, seed=w)
+    idx[0], idx[-1] = 0, tlen - 1
+    want = orc.ig_agi(to_np(idx), orc.fill_splitmix(tlen))
+    pck = torch.empty_like(idx)
+    B.pack(pck, idx)
+    for packed, src in ((False, idx), (True, pck)):
+        tgt = torch.full((n,), 5, dtype=torch.int64, device=dev)
+        B.gather(tgt, src, table, packed=packed, cfg=B.config(B.BUCKET, tile_log2=w, hot=0))
+        assert B.lib().bale_hip_last_strategy() == B.BUCKET
+        assert np.array_equal(to_np(tgt), want), packed
+
+
 def test_gather_bucket_errors_and_ragged(dev, orc):
     """BUCKET gather: out-of-range and foreign-PE requests are counted and
     their slots read 0; a ragged last chunk; regions narrower than default."""
